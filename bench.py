@@ -1,0 +1,156 @@
+"""Benchmark of the NTT engine on one MI355X.
+
+Headline workload: Goldilocks (p = 2^64 - 2^32 + 1) forward NTT of
+N = 2^24 elements, natural order in and out, inputs uniform in [0, p) and
+already resident in HBM.  One step = one transform.  Inputs and outputs
+rotate over 3 buffer pairs (3 x 2 x 128 MiB) so that each step reads its
+input from HBM rather than from the 256 MB Infinity Cache.
+
+Prints ONE JSON line (rank 0):
+  value        elements/s of the 2^24 forward transform
+  roofline     algorithmic bytes (16 N per transform: read N + write N words)
+               per pass launch / average pass-launch duration, vs 8 TB/s
+  cpu_baseline the CPU restatement of the reference's WinterfellFourStep at
+               2^24 (OpenMP over the host cores), timed on a bounded sample
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "omr-ntt_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_device(fn, steps, warmup, torch):
+    """Times `steps` calls of fn() with events on torch's current stream
+    (the stream the engine launches on); returns seconds per call."""
+    for i in range(warmup):
+        fn(i)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for i in range(steps):
+        fn(i)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / 1e3 / steps
+
+
+def cpu_baseline(x, n, budget_s=20.0):
+    """WinterfellFourStep restated in C (oracle/), OpenMP over host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    f = O.GOLDILOCKS
+    d = O.Domain(f, n)
+    times = []
+    out = None
+    t_start = time.time()
+    while len(times) < 10 and time.time() - t_start < budget_s:
+        t0 = time.perf_counter()
+        out = O.encoder("WinterfellFourStep", f, x, dom=d)
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    return {"value": n / med, "unit": "elems/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"WinterfellFourStep (C restatement) Goldilocks 2^{n.bit_length() - 1}, "
+                      f"median of {len(times)} runs"}, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--logn", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the engine has no multi-GPU path "
+              "(each GPU would run its own share; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import omr_ntt as E
+
+    F = E.Goldilocks
+    n = 1 << args.logn
+    sched = E.default_schedule(F, n)
+    npass = len(sched)
+    log(f"bench: Goldilocks 2^{args.logn} forward NTT, schedule {sched}, steps={args.steps} warmup={args.warmup}")
+
+    rng = np.random.default_rng(42)
+    host_x = rng.integers(0, F.p, size=n, dtype=np.uint64)
+    nbuf = 3
+    xs = [E.to_device(host_x if i == 0 else rng.integers(0, F.p, size=n, dtype=np.uint64), F)
+          for i in range(nbuf)]
+    ys = [E.empty_device(n, F) for _ in range(nbuf)]
+
+    def fwd(i):
+        E.ntt_device(xs[i % nbuf], ys[i % nbuf], F)
+
+    sec = time_device(fwd, args.steps, args.warmup, torch)
+    log(f"bench: forward {sec * 1e6:.1f} us/transform")
+
+    # bit-exact check on the GPU: inverse round trip of every rotated buffer
+    back = E.empty_device(n, F)
+    round_trip = True
+    for i in range(nbuf):
+        E.ntt_device(xs[i], ys[i], F)
+        E.intt_device(ys[i], back, F)
+        round_trip &= bool(torch.equal(back, xs[i]))
+
+    def inv(i):
+        E.intt_device(ys[i % nbuf], xs[i % nbuf], F)
+
+    # intt(ntt(x)) == x, so the inverse steps rewrite the inputs unchanged
+    sec_inv = time_device(inv, max(5, args.steps // 2), 3, torch)
+    log(f"bench: inverse {sec_inv * 1e6:.1f} us/transform, round trip ok={round_trip}")
+
+    alg_bytes = 16 * n
+    per_launch = alg_bytes / npass
+    achieved = per_launch / (sec / npass) / 1e9
+    result = {
+        "metric": "Goldilocks 2^24 NTT elems/s & %HBM peak; batched-vs-plain speedup, bit-exact",
+        "value": n / sec,
+        "unit": "elems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks)",
+        "data": "synthetic: uniform in [0, p), numpy PCG64 seed 42, 3 rotating HBM-resident buffers",
+        "config": {"workload": f"Goldilocks forward NTT N=2^{args.logn}, natural order in/out",
+                   "schedule_log_radix": sched, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_stockham_pass (one launch per pass)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": per_launch, "launches_per_step": npass},
+        "inverse_us": sec_inv * 1e6,
+        "bit_exact": {"gpu_round_trip": round_trip},
+    }
+    if not args.no_cpu:
+        cb, cpu_out = cpu_baseline(host_x, n)
+        result["cpu_baseline"] = cb
+        E.ntt_device(xs[0], ys[0], F)
+        result["bit_exact"]["gpu_vs_cpu_restatement"] = bool(
+            np.array_equal(E.from_device(ys[0], F), cpu_out))
+        log(f"bench: cpu baseline {cb['value']:.3e} elems/s on {cb['cores']} threads")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

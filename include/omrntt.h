@@ -1,0 +1,148 @@
+/*
+ * omrntt.h -- C ABI of the MI355X (gfx950) NTT engine.
+ *
+ * Drop-in boundary for the reference crate's operator API (benbencik/omr-ntt):
+ *   trait NttEncoder<F> { fn ntt(&self, buf: &mut [F], domain: &NttDomain<F>);
+ *                         fn name(&self) -> &str; }      (src/encoder.rs:70-74)
+ * plus the domain checks of NttDomain::new (src/encoder.rs:23-52) and the
+ * field set of src/fields.rs:3-17.
+ *
+ * Conventions
+ *  - Field elements cross this boundary as canonical integers in [0, p):
+ *    uint32_t for BabyBear/KoalaBear/TeddyBear, uint64_t for Goldilocks.
+ *  - `omega` is the caller's primitive N-th root of unity (the Rust shim
+ *    passes NttDomain::omega, canonical), so results follow the caller's
+ *    domain exactly.  omr_root_of_unity gives the ark-ff convention
+ *    g^((p-1)/N).
+ *  - Every function validates all arguments before touching the GPU and
+ *    returns an omr_status; nothing panics or aborts.
+ *  - `_device` entry points take device pointers and a hipStream_t (passed as
+ *    void*, NULL = default stream) and are asynchronous on that stream.
+ *    Calls sharing one omr_ctx share its scratch buffers: issue them on one
+ *    stream (or use one context per stream).
+ *  - The host-buffer entry point omr_ntt is synchronous and thread safe
+ *    (it holds the context lock), matching `NttEncoder: Send + Sync`.
+ */
+#ifndef OMRNTT_H
+#define OMRNTT_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  OMR_OK = 0,
+  OMR_E_NULL = 1,         /* null pointer argument */
+  OMR_E_SIZE = 2,         /* N not a power of two   (src/encoder.rs:24) */
+  OMR_E_TWO_ADICITY = 3,  /* log2 N > two-adicity   (src/encoder.rs:28-32) */
+  OMR_E_FIELD = 4,        /* unknown field id */
+  OMR_E_VARIANT = 5,      /* unknown encoder variant */
+  OMR_E_RADIX = 6,        /* variant constraint (lambdaworks_radix4.rs:17-21,
+                             tfhe_stockham_radix8.rs:17-21) */
+  OMR_E_OMEGA = 7,        /* omega is not a primitive N-th root of unity */
+  OMR_E_OUTLEN = 8,       /* invalid partial-output length / subset */
+  OMR_E_ALLOC = 9,        /* device or host allocation failed */
+  OMR_E_DEVICE = 10,      /* HIP runtime or launch error */
+  OMR_E_SCHEDULE = 11,    /* explicit schedule invalid for this N / field */
+  OMR_E_ARG = 12          /* other invalid argument */
+} omr_status;
+
+typedef enum {
+  OMR_BABYBEAR = 0,   /* p = 2^31 - 2^27 + 1, g = 31 (src/fields.rs:4) */
+  OMR_KOALABEAR = 1,  /* p = 2^31 - 2^24 + 1, g = 3  (src/fields.rs:7) */
+  OMR_GOLDILOCKS = 2, /* p = 2^64 - 2^32 + 1, g = 7  (src/fields.rs:10-14) */
+  OMR_TEDDYBEAR = 3   /* p = 2^32 - 2^30 + 1, g = 5  (src/fields.rs:17) */
+} omr_field;
+
+/* Encoder variants: the reference's registry (src/encoders/mod.rs:15-54).
+ * Full variants compute out[k] = sum_j in[j] w^(jk) for all k < N; each maps
+ * to a schedule of the same fused-LDS Stockham engine.  Partial variants
+ * compute out[k] for k < out_len only. */
+typedef enum {
+  OMR_ARK_RADIX2 = 0,              /* src/encoders/ark_radix2.rs:10-24 (tuned schedule) */
+  OMR_DFT = 1,                     /* src/encoders/dft.rs:7-27 (O(N^2) kernel) */
+  OMR_LAMBDA_BOWERS = 2,           /* src/encoders/lambdaworks_bowers.rs:11-173 */
+  OMR_LAMBDA_RADIX4 = 3,           /* src/encoders/lambdaworks_radix4.rs:12-75 (log N even) */
+  OMR_PLONKY3_LAYER_SPLIT = 4,     /* src/encoders/plonky3_radix2_layer_split.rs:16-97 */
+  OMR_WINTERFELL_FOUR_STEP = 5,    /* src/encoders/winterfell_four_step.rs:17-130 */
+  OMR_TFHE_STOCKHAM_RADIX8 = 6,    /* src/encoders/tfhe_stockham_radix8.rs:13-140 (log N % 3 == 0) */
+  OMR_DFT_PARTIAL = 7,             /* src/encoders/dft.rs:29-94 */
+  OMR_TRANSFORM_DECOMPOSITION = 8, /* src/encoders/transform_decomposition.rs:10-102 */
+  OMR_TRANSFORM_DECOMPOSITION_V2 = 9, /* src/encoders/transform_decomposition_v2.rs:8-105 */
+  OMR_VARIANT_COUNT = 10
+} omr_variant;
+
+typedef struct omr_ctx omr_ctx;
+
+/* ---- library / context -------------------------------------------------- */
+const char *omr_status_str(omr_status s);
+const char *omr_version(void);
+/* Creates a context bound to HIP device `device`. */
+omr_status omr_ctx_create(int device, omr_ctx **out);
+omr_status omr_ctx_destroy(omr_ctx *ctx);
+/* Waits for all work the context issued on `stream`. */
+omr_status omr_ctx_synchronize(omr_ctx *ctx, void *stream);
+
+/* ---- fields and domains (src/fields.rs, src/encoder.rs:23-52) ----------- */
+omr_status omr_field_info(int field, uint64_t *modulus, uint64_t *generator, int *two_adicity);
+/* Element width in bytes (4 or 8). */
+int omr_field_bytes(int field);
+/* NttDomain::new checks plus F::get_root_of_unity(N) (src/encoder.rs:23-35). */
+omr_status omr_root_of_unity(int field, uint64_t n, uint64_t *omega);
+/* NttEncoder::name (src/encoder.rs:73); NULL for an unknown variant. */
+const char *omr_variant_name(int variant);
+/* Whether `variant` accepts N = 2^log_n (registry rule, src/encoders/mod.rs:28-46). */
+int omr_variant_supports(int variant, uint32_t log_n);
+
+/* ---- the drop-in: NttEncoder::ntt on a host buffer ---------------------- */
+/* In place on `buf` (n canonical elements).  Full variants overwrite all n;
+ * partial variants (s > 0) write buf[0 .. min(2s, n)) and leave the tail
+ * holding the input, as the reference does (transform_decomposition.rs:96,
+ * dft.rs:88).  Synchronous. */
+omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n, uint64_t omega,
+                   uint64_t s);
+
+/* ---- device-resident entry points --------------------------------------- */
+/* Forward transform of `batch` vectors of n elements (consecutive in memory,
+ * stride n) under `variant`.  d_in may equal d_out.  For partial variants
+ * out_len = min(2s, n) elements are written per vector. */
+omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const void *d_in, void *d_out,
+                          uint64_t n, uint64_t batch, uint64_t omega, uint64_t s, void *stream);
+
+/* Inverse: x[j] = n^-1 sum_k X[k] omega^(-jk).  d_in may equal d_out. */
+omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in, void *d_out, uint64_t n,
+                           uint64_t batch, uint64_t omega, void *stream);
+
+/* Transform under an explicit pass schedule: npass passes of radix
+ * 2^log_radix[i] (sum = log2 n), register radix 2^log_e (2..4; 0 = auto).
+ * inverse != 0 selects the inverse transform.  Used by the autotuner. */
+omr_status omr_ntt_schedule_device(omr_ctx *ctx, int field, const void *d_in, void *d_out,
+                                   uint64_t n, uint64_t batch, uint64_t omega, int inverse,
+                                   const int *log_radix, int npass, int log_e, void *stream);
+
+/* Schedule the engine would use by default for (field, n): writes up to 8
+ * log radices, returns the pass count in *npass. */
+omr_status omr_default_schedule(int field, uint64_t n, int *log_radix, int *npass);
+
+/* Partial output.  mode 0 (prefix): out[k] = X[k] for k < out_len, the
+ * reference's `out[0..min(2s, N)]` with out_len = min(2s, N).  mode 1
+ * (stride subset): out[j] = X[j * n / out_len] for j < out_len, out_len a
+ * power of two dividing n.  method: 0 auto, 1 direct partial DFT,
+ * 2 decomposition with sub-NTT size 2^log_n1 (0 = auto), 3 full transform
+ * then prefix.  d_out holds out_len elements. */
+omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_in, void *d_out, uint64_t n,
+                              uint64_t omega, uint64_t out_len, int mode, int method,
+                              uint32_t log_n1, void *stream);
+
+/* Elementwise device field arithmetic on canonical arrays (test hook):
+ * op 0 add, 1 sub, 2 mul, 3 multiplier-form round trip of a, 4 a * b via
+ * multiplier form. */
+omr_status omr_field_op_device(int field, int op, const void *d_a, const void *d_b, void *d_out,
+                               uint64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMRNTT_H */

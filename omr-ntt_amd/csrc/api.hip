@@ -1,0 +1,665 @@
+// api.hip -- C ABI of the NTT engine: validation, plans, schedules and launches.
+//
+// Host code only; the kernels live in pass_<field>.hip (Stockham passes) and
+// misc_kernels.hip.  See include/omrntt.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/omrntt.h"
+#include "dispatch.h"
+#include "internal.h"
+#include "misc.h"
+
+using namespace omr;
+
+// ---------------------------------------------------------------------------
+// status strings, field facts
+// ---------------------------------------------------------------------------
+extern "C" const char *omr_status_str(omr_status s) {
+  switch (s) {
+    case OMR_OK: return "ok";
+    case OMR_E_NULL: return "null pointer argument";
+    case OMR_E_SIZE: return "NTT size N must be a power of 2";
+    case OMR_E_TWO_ADICITY: return "N requires log2(N) <= TWO_ADICITY (N must divide p-1)";
+    case OMR_E_FIELD: return "unknown field";
+    case OMR_E_VARIANT: return "unknown encoder variant";
+    case OMR_E_RADIX: return "variant constraint violated (radix-4 needs log2 N even, radix-8 needs log2 N divisible by 3)";
+    case OMR_E_OMEGA: return "omega is not a primitive N-th root of unity";
+    case OMR_E_OUTLEN: return "invalid partial output length";
+    case OMR_E_ALLOC: return "allocation failed";
+    case OMR_E_DEVICE: return "HIP device error";
+    case OMR_E_SCHEDULE: return "invalid schedule";
+    case OMR_E_ARG: return "invalid argument";
+  }
+  return "unknown status";
+}
+
+extern "C" const char *omr_version(void) { return "omrntt-gfx950 0.1.0"; }
+
+namespace omr {
+
+static const uint64_t kP[4] = {BabyBear::P, KoalaBear::P, Goldilocks::P, TeddyBear::P};
+static const uint64_t kG[4] = {31, 3, 7, 5};
+static const int kTwoAdicity[4] = {27, 24, 32, 30};
+
+int field_bytes(int f) { return f == kGoldilocks ? 8 : 4; }
+bool field_ok(int f) { return f >= 0 && f <= 3; }
+
+uint64_t fmul_host(int f, uint64_t a, uint64_t b) {
+  return (uint64_t)(((unsigned __int128)a * b) % kP[f]);
+}
+uint64_t fpow_host(int f, uint64_t a, uint64_t e) {
+  uint64_t r = 1;
+  a %= kP[f];
+  while (e) {
+    if (e & 1) r = fmul_host(f, r, a);
+    a = fmul_host(f, a, a);
+    e >>= 1;
+  }
+  return r;
+}
+uint64_t finv_host(int f, uint64_t a) { return fpow_host(f, a, kP[f] - 2); }
+uint64_t to_w_host(int f, uint64_t a) {
+  // multiplier form: Montgomery (a * 2^32 mod p) for 32-bit fields, a for Goldilocks
+  if (f == kGoldilocks) return a;
+  return (uint64_t)((((unsigned __int128)a) << 32) % kP[f]);
+}
+uint64_t field_modulus(int f) { return kP[f]; }
+uint64_t field_generator(int f) { return kG[f]; }
+
+static int ilog2(uint64_t n) { return 63 - __builtin_clzll(n); }
+
+// NttDomain::new checks (src/encoder.rs:24-32) plus a primitive-root check of
+// the caller's omega.
+omr_status validate_domain(int f, uint64_t n, uint64_t omega) {
+  if (!field_ok(f)) return OMR_E_FIELD;
+  if (n == 0 || (n & (n - 1))) return OMR_E_SIZE;
+  if (ilog2(n) > kTwoAdicity[f]) return OMR_E_TWO_ADICITY;
+  if (omega >= kP[f]) return OMR_E_OMEGA;
+  if (fpow_host(f, omega, n) != 1) return OMR_E_OMEGA;
+  if (n > 1 && fpow_host(f, omega, n / 2) == 1) return OMR_E_OMEGA;
+  return OMR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers and the twiddle-table cache
+// ---------------------------------------------------------------------------
+static bool alloc_fail_injected() {
+  const char *e = getenv("OMR_FAIL_ALLOC");
+  return e && e[0] == '1';
+}
+
+omr_status ensure_buf(DevBuf &b, size_t bytes) {
+  if (bytes <= b.bytes) return OMR_OK;
+  if (alloc_fail_injected()) return OMR_E_ALLOC;
+  if (b.p) {
+    if (hipDeviceSynchronize() != hipSuccess) return OMR_E_DEVICE;
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return OMR_E_ALLOC;
+  }
+  b.bytes = want;
+  return OMR_OK;
+}
+
+void free_buf(DevBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+// Two-level power table of `root` (order 2^logN) plus the radix root table.
+omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const TwTables **out) {
+  auto key = std::make_tuple(f, logN, root);
+  auto it = ctx->tables.find(key);
+  if (it != ctx->tables.end()) {
+    *out = &it->second;
+    return OMR_OK;
+  }
+  if (ctx->tables.size() >= 128) {
+    if (hipDeviceSynchronize() != hipSuccess) return OMR_E_DEVICE;
+    for (auto &kv : ctx->tables) {
+      (void)hipFree(kv.second.lo);
+      (void)hipFree(kv.second.hi);
+      (void)hipFree(kv.second.root);
+    }
+    ctx->tables.clear();
+  }
+  if (alloc_fail_injected()) return OMR_E_ALLOC;
+  const int es = field_bytes(f);
+  TwTables t{};
+  t.lo_bits = std::min<uint32_t>(logN, 12);
+  const uint64_t nlo = 1ull << t.lo_bits, nhi = 1ull << (logN - t.lo_bits);
+  t.log_rmax = std::min<uint32_t>(logN, (uint32_t)max_log_radix(es));
+  const uint64_t nroot = 1ull << t.log_rmax;
+  std::vector<uint64_t> lo(nlo), hi(nhi), rt(nroot);
+  uint64_t cur = 1;
+  for (uint64_t i = 0; i < nlo; i++) {
+    lo[i] = to_w_host(f, cur);
+    cur = fmul_host(f, cur, root);
+  }
+  const uint64_t step_hi = cur;  // root^(2^lo_bits)
+  cur = 1;
+  for (uint64_t i = 0; i < nhi; i++) {
+    hi[i] = to_w_host(f, cur);
+    cur = fmul_host(f, cur, step_hi);
+  }
+  const uint64_t step_rt = fpow_host(f, root, 1ull << (logN - t.log_rmax));
+  cur = 1;
+  for (uint64_t i = 0; i < nroot; i++) {
+    rt[i] = to_w_host(f, cur);
+    cur = fmul_host(f, cur, step_rt);
+  }
+  auto upload = [&](std::vector<uint64_t> &v, void **dp) -> omr_status {
+    std::vector<uint32_t> v32;
+    const void *src = v.data();
+    if (es == 4) {
+      v32.resize(v.size());
+      for (size_t i = 0; i < v.size(); i++) v32[i] = (uint32_t)v[i];
+      src = v32.data();
+    }
+    if (hipMalloc(dp, v.size() * es) != hipSuccess) {
+      (void)hipGetLastError();
+      return OMR_E_ALLOC;
+    }
+    if (hipMemcpy(*dp, src, v.size() * es, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
+    return OMR_OK;
+  };
+  omr_status st;
+  if ((st = upload(lo, &t.lo)) != OMR_OK) return st;
+  if ((st = upload(hi, &t.hi)) != OMR_OK) return st;
+  if ((st = upload(rt, &t.root)) != OMR_OK) return st;
+  auto res = ctx->tables.emplace(key, t);
+  *out = &res.first->second;
+  return OMR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// schedules
+// ---------------------------------------------------------------------------
+// Largest pass radix whose tile still has >= 16 columns (64-128 B runs).
+static int coalesced_max_log_radix(int es) {
+  return ilog2((uint64_t)(kLdsBudget / es / 16));
+}
+
+// Split `total` radix-2 layers into `parts` balanced pieces (larger first).
+static void balanced(int total, int parts, std::vector<int> &out) {
+  int base = total / parts, rem = total % parts;
+  for (int i = 0; i < parts; i++) out.push_back(base + (i < rem ? 1 : 0));
+}
+
+void default_schedule(int f, uint32_t logN, Schedule &s) {
+  s.logR.clear();
+  s.logE = 0;
+  if (logN == 0) return;
+  const int es = field_bytes(f);
+  if ((int)logN <= max_log_radix(es)) {
+    s.logR.push_back(logN);
+    return;
+  }
+  const int cm = coalesced_max_log_radix(es);
+  const int parts = (logN + cm - 1) / cm;
+  balanced(logN, parts, s.logR);
+}
+
+// Split `total` layers into pieces no larger than `cap`, each a multiple of `unit`.
+static bool split_units(int total, int cap, int unit, std::vector<int> &out) {
+  if (total % unit) return false;
+  int units = total / unit, capu = std::max(1, cap / unit);
+  int parts = (units + capu - 1) / capu;
+  int base = units / parts, rem = units % parts;
+  for (int i = 0; i < parts; i++) out.push_back((base + (i < rem ? 1 : 0)) * unit);
+  return true;
+}
+
+omr_status variant_schedule(int f, int variant, uint32_t logN, Schedule &s) {
+  const int es = field_bytes(f);
+  const int cap = max_log_radix(es);
+  const int ccap = coalesced_max_log_radix(es);
+  s.logR.clear();
+  s.logE = 0;
+  if (logN == 0) return OMR_OK;
+  switch (variant) {
+    case OMR_ARK_RADIX2:
+      default_schedule(f, logN, s);
+      return OMR_OK;
+    case OMR_LAMBDA_BOWERS:  // two fused radix-2 layers per register step
+      default_schedule(f, logN, s);
+      s.logE = 2;
+      for (int &r : s.logR)
+        if (r < 2) s.logE = 0;
+      return OMR_OK;
+    case OMR_LAMBDA_RADIX4:  // radix-4 digits only (lambdaworks_radix4.rs:17-21)
+      if (logN % 2) return OMR_E_RADIX;
+      split_units(logN, (int)logN <= cap ? cap : ccap, 2, s.logR);
+      s.logE = 2;
+      return OMR_OK;
+    case OMR_TFHE_STOCKHAM_RADIX8:  // radix-8 digits only (tfhe_stockham_radix8.rs:17-21)
+      if (logN % 3) return OMR_E_RADIX;
+      split_units(logN, (int)logN <= cap ? cap : ccap + 1, 3, s.logR);
+      s.logE = 3;
+      return OMR_OK;
+    case OMR_WINTERFELL_FOUR_STEP: {  // inner = 2^floor(logN/2) first (winterfell_four_step.rs:40-44)
+      int inner = logN / 2, outer = logN - inner;
+      if (inner) split_units(inner, cap, 1, s.logR);
+      split_units(outer, cap, 1, s.logR);
+      return OMR_OK;
+    }
+    case OMR_PLONKY3_LAYER_SPLIT: {  // mid = ceil(logN/2) (plonky3_radix2_layer_split.rs:33)
+      int mid = (logN + 1) / 2, rest = logN - mid;
+      split_units(mid, cap, 1, s.logR);
+      if (rest) split_units(rest, cap, 1, s.logR);
+      return OMR_OK;
+    }
+    default:
+      return OMR_E_VARIANT;
+  }
+}
+
+static int auto_log_e(int logR) {
+  if (logR <= 2) return logR;
+  return std::min(4, logR);
+}
+
+// ---------------------------------------------------------------------------
+// the multi-pass transform
+// ---------------------------------------------------------------------------
+omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
+                         uint64_t root, const Schedule &sch, uint64_t batch, uint64_t in_stride,
+                         uint64_t out_stride, const XformOpts &o, hipStream_t stream) {
+  const int es = field_bytes(f);
+  const uint64_t N = 1ull << logN;
+  if (batch == 0) return OMR_OK;
+  if (batch > 65535) return OMR_E_ARG;
+  if (logN == 0) {  // size-1 transform: identity (times the scale)
+    const TwTables *tt;
+    (void)tt;
+    for (uint64_t b = 0; b < batch; b++) {
+      omr_status st = launch_scale(f, (const char *)d_in + b * in_stride * es,
+                                   (char *)d_out + b * out_stride * es, 1, o.scale_w,
+                                   o.scale ? 1 : 0, stream);
+      if (st != OMR_OK) return st;
+    }
+    return OMR_OK;
+  }
+  int sum = 0;
+  for (int r : sch.logR) {
+    if (r < 1 || r > max_log_radix(es)) return OMR_E_SCHEDULE;
+    sum += r;
+  }
+  if (sum != (int)logN || sch.logR.empty() || sch.logR.size() > 8) return OMR_E_SCHEDULE;
+  const int np = (int)sch.logR.size();
+
+  const TwTables *tab;
+  omr_status st = get_tables(ctx, f, logN, root, &tab);
+  if (st != OMR_OK) return st;
+
+  // ping-pong buffers X_0 = in, X_np = out, X_1..X_np-1 from scratch
+  std::vector<const void *> X(np + 1);
+  X[0] = d_in;
+  X[np] = d_out;
+  const size_t vec_bytes = (size_t)N * es * batch;
+  // d_out doubles as a ping-pong buffer only when it holds whole vectors
+  const bool use_out = (o.store_limit >= N) && ((d_in != d_out) || (np % 2 == 0));
+  if (np >= 2) {
+    if ((st = ensure_buf(ctx->scratch[0], vec_bytes)) != OMR_OK) return st;
+    if (!use_out && np >= 3)
+      if ((st = ensure_buf(ctx->scratch[1], vec_bytes)) != OMR_OK) return st;
+  }
+  for (int i = 1; i < np; i++) {
+    if (use_out) X[i] = ((np - i) % 2 == 0) ? d_out : ctx->scratch[0].p;
+    else X[i] = ((np - i) % 2 == 1) ? ctx->scratch[0].p : ctx->scratch[1].p;
+  }
+  // with use_out and an out-of-place call whose out vectors are strided, the
+  // intermediates must be dense N-element vectors
+  if ((use_out && np >= 2 && out_stride != N) || (!use_out && np >= 2)) {
+    if ((st = ensure_buf(ctx->scratch[1], vec_bytes)) != OMR_OK) return st;
+    for (int i = 1; i < np; i++) X[i] = ((np - i) % 2 == 1) ? ctx->scratch[0].p : ctx->scratch[1].p;
+  }
+
+  int consumed = 0;
+  for (int i = 0; i < np; i++) {
+    const int logR = sch.logR[i];
+    consumed += logR;
+    int logE = sch.logE;
+    if (!has_pass_kernel(logR, logE, es)) logE = auto_log_e(logR);
+    if (!has_pass_kernel(logR, logE, es)) return OMR_E_SCHEDULE;
+    const int cmax = pass_cmax(logR, es);
+    const int logC = std::min<int>(ilog2((uint64_t)cmax), (int)logN - logR);
+    PassArgs a{};
+    a.src = X[i];
+    a.dst = (void *)X[i + 1];
+    a.tw_lo = tab->lo;
+    a.tw_hi = tab->hi;
+    a.root = tab->root;
+    a.lo_bits = tab->lo_bits;
+    a.root_shift = tab->log_rmax - logR;
+    a.logN = logN;
+    a.logS = logN - consumed;
+    a.logC = logC;
+    a.vec_stride_in = (i == 0) ? in_stride : N;
+    a.vec_stride_out = (i == np - 1) ? out_stride : N;
+    a.store_limit = (i == np - 1) ? o.store_limit : ~0ull;
+    a.flags = 0;
+    if (i > 0) a.flags |= kTwiddle;
+    if (i == np - 1 && o.scale) {
+      a.flags |= kScale;
+      a.scale_w = o.scale_w;
+    }
+    if (i == 0) {
+      if (o.zero_tail) {
+        a.flags |= kZeroTail;
+        a.logValid = o.log_valid;
+      }
+      a.pre_lo = o.pre_lo;
+      a.pre_hi = o.pre_hi;
+    }
+    dim3 block((1u << logC) << (logR - logE));
+    dim3 grid((unsigned)((N >> logR) >> logC), (unsigned)batch);
+    if (launch_pass(f, logR, logE, a, grid, block, stream) != hipSuccess) return OMR_E_DEVICE;
+  }
+  return OMR_OK;
+}
+
+}  // namespace omr
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" omr_status omr_ctx_create(int device, omr_ctx **out) {
+  if (!out) return OMR_E_NULL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    return OMR_E_DEVICE;
+  }
+  if (hipSetDevice(device) != hipSuccess) return OMR_E_DEVICE;
+  omr_ctx *c = new (std::nothrow) omr_ctx();
+  if (!c) return OMR_E_ALLOC;
+  c->device = device;
+  *out = c;
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
+  if (!ctx) return OMR_E_NULL;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  for (auto &kv : ctx->tables) {
+    (void)hipFree(kv.second.lo);
+    (void)hipFree(kv.second.hi);
+    (void)hipFree(kv.second.root);
+  }
+  for (auto &b : ctx->scratch) free_buf(b);
+  free_buf(ctx->io[0]);
+  free_buf(ctx->io[1]);
+  free_buf(ctx->part);
+  delete ctx;
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_ctx_synchronize(omr_ctx *ctx, void *stream) {
+  if (!ctx) return OMR_E_NULL;
+  return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+}
+
+extern "C" omr_status omr_field_info(int field, uint64_t *modulus, uint64_t *generator,
+                                     int *two_adicity) {
+  if (!field_ok(field)) return OMR_E_FIELD;
+  if (modulus) *modulus = kP[field];
+  if (generator) *generator = kG[field];
+  if (two_adicity) *two_adicity = kTwoAdicity[field];
+  return OMR_OK;
+}
+
+extern "C" int omr_field_bytes(int field) { return field_ok(field) ? field_bytes(field) : 0; }
+
+extern "C" omr_status omr_root_of_unity(int field, uint64_t n, uint64_t *omega) {
+  if (!omega) return OMR_E_NULL;
+  if (!field_ok(field)) return OMR_E_FIELD;
+  if (n == 0 || (n & (n - 1))) return OMR_E_SIZE;
+  if (ilog2(n) > kTwoAdicity[field]) return OMR_E_TWO_ADICITY;
+  *omega = fpow_host(field, kG[field], (kP[field] - 1) / n);
+  return OMR_OK;
+}
+
+static const char *kVariantNames[OMR_VARIANT_COUNT] = {
+    "ArkRadix2",          "Dft",
+    "LambdaBowers",       "LambdaRadix4",
+    "Plonky3Radix2LayerSplit", "WinterfellFourStep",
+    "TfheStockhamRadix8", "DftPartial",
+    "TransformDecomposition", "TransformDecompositionV2"};
+
+extern "C" const char *omr_variant_name(int variant) {
+  if (variant < 0 || variant >= OMR_VARIANT_COUNT) return nullptr;
+  return kVariantNames[variant];
+}
+
+extern "C" int omr_variant_supports(int variant, uint32_t log_n) {
+  if (variant == OMR_LAMBDA_RADIX4) return log_n % 2 == 0;
+  if (variant == OMR_TFHE_STOCKHAM_RADIX8) return log_n % 3 == 0;
+  return variant >= 0 && variant < OMR_VARIANT_COUNT;
+}
+
+extern "C" omr_status omr_default_schedule(int field, uint64_t n, int *log_radix, int *npass) {
+  if (!log_radix || !npass) return OMR_E_NULL;
+  if (!field_ok(field)) return OMR_E_FIELD;
+  if (n == 0 || (n & (n - 1))) return OMR_E_SIZE;
+  if (ilog2(n) > kTwoAdicity[field]) return OMR_E_TWO_ADICITY;
+  Schedule s;
+  const uint32_t logN = ilog2(n);
+  if (!tuned_schedule(field, logN, s)) default_schedule(field, logN, s);
+  *npass = (int)s.logR.size();
+  for (size_t i = 0; i < s.logR.size(); i++) log_radix[i] = s.logR[i];
+  return OMR_OK;
+}
+
+static bool is_partial_variant(int v) {
+  return v == OMR_DFT_PARTIAL || v == OMR_TRANSFORM_DECOMPOSITION || v == OMR_TRANSFORM_DECOMPOSITION_V2;
+}
+
+static uint64_t next_pow2(uint64_t x) {
+  uint64_t r = 1;
+  while (r < x) r <<= 1;
+  return r;
+}
+
+namespace omr {
+
+// Full forward transform of `batch` dense vectors under `variant`.
+omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void *d_out,
+                        uint32_t logN, uint64_t batch, uint64_t omega, hipStream_t stream) {
+  const int es = field_bytes(f);
+  const uint64_t N = 1ull << logN;
+  if (variant == OMR_DFT) {
+    const void *src = d_in;
+    if (d_in == d_out) {
+      omr_status st = ensure_buf(ctx->scratch[0], (size_t)N * batch * es);
+      if (st != OMR_OK) return st;
+      if (hipMemcpyAsync(ctx->scratch[0].p, d_in, (size_t)N * batch * es, hipMemcpyDeviceToDevice,
+                         stream) != hipSuccess)
+        return OMR_E_DEVICE;
+      src = ctx->scratch[0].p;
+    }
+    const TwTables *tab;
+    omr_status st = get_tables(ctx, f, logN, omega, &tab);
+    if (st != OMR_OK) return st;
+    for (uint64_t b = 0; b < batch; b++) {
+      st = launch_dft_full(f, (const char *)src + b * N * es, (char *)d_out + b * N * es, N, logN,
+                           tab, stream);
+      if (st != OMR_OK) return st;
+    }
+    return OMR_OK;
+  }
+  Schedule s;
+  omr_status st = variant_schedule(f, variant, logN, s);
+  if (st != OMR_OK) return st;
+  if (variant == OMR_ARK_RADIX2) tuned_schedule(f, logN, s);
+  XformOpts o;
+  return run_transform(ctx, f, d_in, d_out, logN, omega, s, batch, N, N, o, stream);
+}
+
+}  // namespace omr
+
+extern "C" omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const void *d_in,
+                                     void *d_out, uint64_t n, uint64_t batch, uint64_t omega,
+                                     uint64_t s, void *stream) {
+  if (variant < 0 || variant >= OMR_VARIANT_COUNT) return OMR_E_VARIANT;
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  const uint32_t logN = ilog2(n);
+  if (!omr_variant_supports(variant, logN)) return OMR_E_RADIX;
+  if (!ctx || !d_in || !d_out) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  hipStream_t hs = (hipStream_t)stream;
+  if (!is_partial_variant(variant)) return forward_full(ctx, field, variant, d_in, d_out, logN, batch, omega, hs);
+  const uint64_t out_len = std::min<uint64_t>(2 * s, n);
+  if (out_len == 0) return OMR_OK;
+  const int es = field_bytes(field);
+  int method = 2;
+  uint32_t log_n1 = 0;
+  if (variant == OMR_DFT_PARTIAL) method = 1;
+  else {
+    // reference rule: n1 = nextpow2(2 out_len) clamped to [2, N]
+    // (transform_decomposition.rs:38-39); capped at the largest one-pass radix
+    uint64_t n1 = std::min<uint64_t>(std::max<uint64_t>(next_pow2(2 * out_len), 2), n);
+    log_n1 = std::min<uint32_t>(ilog2(n1), (uint32_t)max_log_radix(es));
+  }
+  for (uint64_t b = 0; b < batch; b++) {
+    const char *in = (const char *)d_in + b * n * es;
+    char *out = (char *)d_out + b * n * es;
+    st = partial_impl(ctx, field, in, out, logN, omega, out_len, 0, method, log_n1, hs);
+    if (st != OMR_OK) return st;
+  }
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in, void *d_out,
+                                      uint64_t n, uint64_t batch, uint64_t omega, void *stream) {
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (!ctx || !d_in || !d_out) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const uint32_t logN = ilog2(n);
+  Schedule s;
+  if (!tuned_schedule(field, logN, s)) default_schedule(field, logN, s);
+  XformOpts o;
+  o.scale = true;
+  o.scale_w = to_w_host(field, finv_host(field, n % kP[field]));
+  return run_transform(ctx, field, d_in, d_out, logN, finv_host(field, omega), s, batch, n, n, o,
+                       (hipStream_t)stream);
+}
+
+extern "C" omr_status omr_ntt_schedule_device(omr_ctx *ctx, int field, const void *d_in,
+                                              void *d_out, uint64_t n, uint64_t batch,
+                                              uint64_t omega, int inverse, const int *log_radix,
+                                              int npass, int log_e, void *stream) {
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (!log_radix) return OMR_E_NULL;
+  if (npass < 1 || npass > 8) return OMR_E_SCHEDULE;
+  {
+    int sum = 0;
+    for (int i = 0; i < npass; i++) {
+      if (log_radix[i] < 1 || log_radix[i] > max_log_radix(field_bytes(field))) return OMR_E_SCHEDULE;
+      sum += log_radix[i];
+    }
+    if (sum != ilog2(n)) return OMR_E_SCHEDULE;
+  }
+  if (!ctx || !d_in || !d_out) return OMR_E_NULL;
+  if (log_e != 0 && (log_e < 1 || log_e > 4)) return OMR_E_SCHEDULE;
+  Schedule s;
+  for (int i = 0; i < npass; i++) s.logR.push_back(log_radix[i]);
+  s.logE = log_e;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const uint32_t logN = ilog2(n);
+  XformOpts o;
+  uint64_t root = omega;
+  if (inverse) {
+    root = finv_host(field, omega);
+    o.scale = true;
+    o.scale_w = to_w_host(field, finv_host(field, n % kP[field]));
+  }
+  return run_transform(ctx, field, d_in, d_out, logN, root, s, batch, n, n, o, (hipStream_t)stream);
+}
+
+extern "C" omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_in, void *d_out,
+                                         uint64_t n, uint64_t omega, uint64_t out_len, int mode,
+                                         int method, uint32_t log_n1, void *stream) {
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (out_len == 0 || out_len > n) return OMR_E_OUTLEN;
+  if (mode != 0 && mode != 1) return OMR_E_ARG;
+  if (mode == 1 && (out_len & (out_len - 1))) return OMR_E_OUTLEN;
+  if (method < 0 || method > 3) return OMR_E_ARG;
+  const int es = field_bytes(field);
+  if (method == 2 && log_n1 != 0 && (log_n1 > ilog2(n) || (int)log_n1 > max_log_radix(es)))
+    return OMR_E_SCHEDULE;
+  if (!ctx || !d_in || !d_out) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  return partial_impl(ctx, field, d_in, d_out, ilog2(n), omega, out_len, mode, method, log_n1,
+                      (hipStream_t)stream);
+}
+
+extern "C" omr_status omr_field_op_device(int field, int op, const void *d_a, const void *d_b,
+                                          void *d_out, uint64_t n, void *stream) {
+  if (!d_a || !d_b || !d_out) return OMR_E_NULL;
+  if (!field_ok(field)) return OMR_E_FIELD;
+  if (op < 0 || op > 4) return OMR_E_ARG;
+  return launch_field_op(field, op, d_a, d_b, d_out, n, (hipStream_t)stream);
+}
+
+// NttEncoder::ntt on a host buffer (src/encoder.rs:70-74): copy in, run, copy out.
+extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n,
+                              uint64_t omega, uint64_t s) {
+  if (variant < 0 || variant >= OMR_VARIANT_COUNT) return OMR_E_VARIANT;
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  const uint32_t logN = ilog2(n);
+  if (!omr_variant_supports(variant, logN)) return OMR_E_RADIX;
+  if (!ctx || !buf) return OMR_E_NULL;
+  const int es = field_bytes(field);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  if ((st = ensure_buf(ctx->io[0], (size_t)n * es)) != OMR_OK) return st;
+  if (hipMemcpy(ctx->io[0].p, buf, (size_t)n * es, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
+  uint64_t copy_back = n;
+  void *result = ctx->io[0].p;
+  if (!is_partial_variant(variant)) {
+    st = forward_full(ctx, field, variant, ctx->io[0].p, ctx->io[0].p, logN, 1, omega, nullptr);
+  } else {
+    copy_back = std::min<uint64_t>(2 * s, n);
+    if (copy_back == 0) return OMR_OK;
+    if ((st = ensure_buf(ctx->io[1], (size_t)copy_back * es)) != OMR_OK) return st;
+    int method = variant == OMR_DFT_PARTIAL ? 1 : 2;
+    uint32_t log_n1 = 0;
+    if (method == 2) {
+      uint64_t n1 = std::min<uint64_t>(std::max<uint64_t>(next_pow2(2 * copy_back), 2), n);
+      log_n1 = std::min<uint32_t>(ilog2(n1), (uint32_t)max_log_radix(es));
+    }
+    st = partial_impl(ctx, field, ctx->io[0].p, ctx->io[1].p, logN, omega, copy_back, 0, method,
+                      log_n1, nullptr);
+    result = ctx->io[1].p;
+  }
+  if (st != OMR_OK) return st;
+  if (hipMemcpy(buf, result, (size_t)copy_back * es, hipMemcpyDeviceToHost) != hipSuccess) return OMR_E_DEVICE;
+  return OMR_OK;
+}

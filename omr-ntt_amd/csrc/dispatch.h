@@ -1,0 +1,39 @@
+// dispatch.h -- host-side view of the compiled pass kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine.h"
+
+namespace omr {
+
+// Largest pass radix compiled per element size (tile data <= kLdsBudget).
+constexpr int max_log_radix(int esize) { return esize == 8 ? 12 : 13; }
+
+// Register radices compiled for a pass of radix 2^logR.
+inline bool has_pass_kernel(int logR, int logE, int esize) {
+  if (logR < 1 || logR > max_log_radix(esize)) return false;
+  if (logE < 1 || logE > 4 || logE > logR) return false;
+  if (logE == 1) return logR <= 2;
+  return true;
+}
+
+// Columns per tile the kernel for (esize, logR) was compiled for.
+inline int pass_cmax(int logR, int esize) {
+  int c = (kLdsBudget / esize) >> logR;
+  if (c < 1) c = 1;
+  if (c > 32) c = 32;
+  return c;
+}
+
+// Launches one Stockham pass; returns hipSuccess or the launch error.
+hipError_t launch_pass(int field, int logR, int logE, const PassArgs &a, dim3 grid, dim3 block,
+                       hipStream_t stream);
+
+// Per-field entry points (one translation unit each).
+hipError_t launch_pass_babybear(int logR, int logE, const PassArgs &a, dim3 g, dim3 b, hipStream_t s);
+hipError_t launch_pass_koalabear(int logR, int logE, const PassArgs &a, dim3 g, dim3 b, hipStream_t s);
+hipError_t launch_pass_goldilocks(int logR, int logE, const PassArgs &a, dim3 g, dim3 b, hipStream_t s);
+hipError_t launch_pass_teddybear(int logR, int logE, const PassArgs &a, dim3 g, dim3 b, hipStream_t s);
+
+}  // namespace omr

@@ -1,0 +1,248 @@
+// engine.h -- LDS-fused Stockham pass kernel for gfx950.
+//
+// An N-point NTT (N = 2^logN, natural order in and out) runs as a short
+// sequence of HBM passes.  Pass i has radix R = 2^LOGR, stride s and
+// L = N / (R s) groups; it maps
+//   dst[k (N/R) + p s + j] = sum_k' w_R^(k k') * w_N^(k' p s) * src[(p R + k') s + j]
+// for p < L, j < s, k < R: the Stockham autosort step that the reference's
+// radix-8 encoder takes per radix-2^3 layer group
+// (src/encoders/tfhe_stockham_radix8.rs:80-139), here with R up to 2^12 so
+// that a 2^24 transform needs 2 or 3 passes instead of 8.
+//
+// One workgroup owns a tile of C consecutive "columns" q = p s + j (all R
+// rows k).  Stores are always C-element contiguous runs; loads are C-element
+// runs when s >= C, and one contiguous block of C*R elements otherwise (staged
+// through LDS).  Inside the tile the R-point DFT is itself a Stockham sequence
+// of register radix-E DFTs with LDS exchanges between them.
+#pragma once
+#include "field.h"
+
+namespace omr {
+
+struct PassArgs {
+  const void *src;
+  void *dst;
+  const void *tw_lo;   // w_N^e, e < 2^lo_bits                     (multiplier form)
+  const void *tw_hi;   // w_N^(e 2^lo_bits), e < N / 2^lo_bits     (multiplier form)
+  const void *root;    // w_Rmax^m, m < Rmax                        (multiplier form)
+  const void *pre_lo;  // optional input-position multiplier h^i:   h^(i mod 2^lo_bits)
+  const void *pre_hi;  //                                            h^((i >> lo_bits) 2^lo_bits)
+  uint64_t vec_stride_in;   // elements between consecutive input vectors (blockIdx.y)
+  uint64_t vec_stride_out;  // elements between consecutive output vectors
+  uint64_t scale_w;         // multiplier applied to every output when kScale
+  uint64_t store_limit;     // only outputs with index < store_limit are stored
+  uint32_t logN, logS, logC, lo_bits, root_shift;
+  uint32_t flags;     // kTwiddle | kScale | kZeroTail
+  uint32_t logValid;  // with kZeroTail: input positions >= 2^logValid read as zero
+};
+enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u };
+
+template <int LOGX>
+__device__ __forceinline__ constexpr int brev_c(int i) {
+  int r = 0;
+  for (int b = 0; b < LOGX; b++) r |= ((i >> b) & 1) << (LOGX - 1 - b);
+  return r;
+}
+
+// In-register DFT of size X = 2^LOGX on x[OFF .. OFF+X), natural order in
+// and out.  wE[k] = w_E^k for the register radix E = 2^LOGE; the sub-DFT
+// uses w_X = w_E^(E/X).  Radix-2 decimation in frequency followed by a
+// compile-time register permutation (free).
+template <class F, int LOGE, int LOGX, int OFF>
+__device__ __forceinline__ void dft_reg(typename F::T (&x)[1 << LOGE],
+                                        const typename F::T (&wE)[(1 << LOGE) / 2 + 1]) {
+  using T = typename F::T;
+  constexpr int X = 1 << LOGX;
+  constexpr int E = 1 << LOGE;
+#pragma unroll
+  for (int len = X; len >= 2; len >>= 1) {
+    const int half = len >> 1;
+#pragma unroll
+    for (int b = 0; b < X; b += len) {
+#pragma unroll
+      for (int j = 0; j < half; j++) {
+        T u = x[OFF + b + j], v = x[OFF + b + j + half];
+        x[OFF + b + j] = F::add(u, v);
+        T d = F::sub(u, v);
+        x[OFF + b + j + half] = (j == 0) ? d : F::mulw(d, wE[j * (E / len)]);
+      }
+    }
+  }
+  T y[X];
+#pragma unroll
+  for (int i = 0; i < X; i++) y[brev_c<LOGX>(i)] = x[OFF + i];
+#pragma unroll
+  for (int i = 0; i < X; i++) x[OFF + i] = y[i];
+}
+
+// E / X independent sub-DFTs of size X = 2^LOGX on consecutive register groups.
+template <class F, int LOGE, int LOGX, int G>
+struct GroupDft {
+  __device__ __forceinline__ static void run(typename F::T (&x)[1 << LOGE],
+                                             const typename F::T (&wE)[(1 << LOGE) / 2 + 1]) {
+    GroupDft<F, LOGE, LOGX, G - 1>::run(x, wE);
+    dft_reg<F, LOGE, LOGX, (G - 1) << LOGX>(x, wE);
+  }
+};
+template <class F, int LOGE, int LOGX>
+struct GroupDft<F, LOGE, LOGX, 0> {
+  __device__ __forceinline__ static void run(typename F::T (&)[1 << LOGE],
+                                             const typename F::T (&)[(1 << LOGE) / 2 + 1]) {}
+};
+
+constexpr int kLdsBudget = 32768;  // bytes of tile data per workgroup
+
+template <class F, int LOGR, int LOGE>
+struct PassKernel {
+  using T = typename F::T;
+  static constexpr int R = 1 << LOGR;
+  static constexpr int E = 1 << LOGE;
+  static constexpr int RE = R / E;
+  static constexpr int NSTAGE = (LOGR + LOGE - 1) / LOGE;
+  static constexpr int LOGLAST = LOGR - (NSTAGE - 1) * LOGE;  // radix of the last in-tile stage
+  static constexpr int CMAX0 = (kLdsBudget / (int)sizeof(T)) >> LOGR;
+  static constexpr int CMAX = CMAX0 < 1 ? 1 : (CMAX0 > 32 ? 32 : CMAX0);
+  static constexpr int MAXT = CMAX * RE;
+  static constexpr int LDS_N = CMAX * R;
+};
+
+// Padded LDS index: one spare slot every 32 elements spreads the power-of-two
+// strides of the late in-tile stages over the banks.
+__device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
+
+// In-tile stage ST >= 1 of the R-point Stockham: reads the previous stage's
+// outputs from LDS, applies w_R^(k' p s), runs the register DFTs.
+template <class F, int LOGR, int LOGE, int ST>
+__device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
+                                           const typename F::T (&wE)[(1 << LOGE) / 2 + 1],
+                                           typename F::T *lds, const typename F::T *root,
+                                           uint32_t t, uint32_t c, uint32_t logC,
+                                           uint32_t root_shift) {
+  using K = PassKernel<F, LOGR, LOGE>;
+  using T = typename F::T;
+  if constexpr (ST < K::NSTAGE) {
+    constexpr bool LAST = (ST == K::NSTAGE - 1);
+    constexpr int LOGEI = LAST ? K::LOGLAST : LOGE;
+    constexpr int EI = 1 << LOGEI;
+    constexpr int LOGS = LOGR - ST * LOGE - LOGEI;  // in-tile stride of this stage
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < K::E; m++) lds[lpad(((t + m * K::RE) << logC) + c)] = x[m];
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < K::E / EI; g++) {
+      const uint32_t tt = t + g * K::RE;
+      const uint32_t ps = (tt >> LOGS) << LOGS;  // p * s within the tile
+      const uint32_t jj = tt & ((1u << LOGS) - 1);
+#pragma unroll
+      for (int kk = 0; kk < EI; kk++) {
+        const uint32_t row = ps * EI + ((uint32_t)kk << LOGS) + jj;
+        T v = lds[lpad((row << logC) + c)];
+        const uint32_t e = (uint32_t)kk * ps;
+        if (kk != 0 && e != 0) v = F::mulw(v, root[e << root_shift]);
+        x[g * EI + kk] = v;
+      }
+    }
+    GroupDft<F, LOGE, LOGEI, K::E / EI>::run(x, wE);
+    tile_stage<F, LOGR, LOGE, ST + 1>(x, wE, lds, root, t, c, logC, root_shift);
+  }
+}
+
+template <class F, int LOGR, int LOGE>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, PassKernel<F, LOGR, LOGE>::MAXT)))
+k_stockham_pass(PassArgs a) {
+  using K = PassKernel<F, LOGR, LOGE>;
+  using T = typename F::T;
+  constexpr int R = K::R, E = K::E, RE = K::RE;
+  __shared__ T lds[K::LDS_N + K::LDS_N / 32 + 1];
+
+  const uint32_t logC = a.logC;
+  const uint32_t C = 1u << logC;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t c = tid & (C - 1), t = tid >> logC;
+  const uint32_t logS = a.logS;
+  const uint64_t S = 1ull << logS;
+  const T *__restrict__ src = (const T *)a.src + (uint64_t)blockIdx.y * a.vec_stride_in;
+  T *__restrict__ dst = (T *)a.dst + (uint64_t)blockIdx.y * a.vec_stride_out;
+  const T *__restrict__ root = (const T *)a.root;
+
+  const uint64_t q = ((uint64_t)blockIdx.x << logC) + c;
+  const uint64_t p = q >> logS, j = q & (S - 1);
+
+  T wE[E / 2 + 1];
+#pragma unroll
+  for (int k = 0; k <= E / 2; k++) wE[k] = root[((uint32_t)k * RE) << a.root_shift];
+
+  // input position of row k in this column: (p R + k) s + j
+  const uint64_t pos0 = ((p << LOGR) << logS) + j;
+  const bool zero_tail = (a.flags & kZeroTail) != 0;
+  const uint64_t valid = zero_tail ? (1ull << a.logValid) : ~0ull;
+
+  T x[E];
+  if (S >= C) {
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const uint64_t pos = pos0 + ((uint64_t)(t + m * RE) << logS);
+      x[m] = (pos < valid) ? src[pos] : (T)0;
+    }
+  } else {
+    // The tile is the contiguous block [q0 R, (q0 + C) R): element
+    // i = (pp R + k) S + jj belongs to column pp S + jj, row k.
+    const uint64_t blk0 = (uint64_t)blockIdx.x << (logC + LOGR);
+    const uint32_t total = C << LOGR, nthr = C * RE;
+    for (uint32_t i = tid; i < total; i += nthr) {
+      const uint32_t jj = i & (uint32_t)(S - 1);
+      const uint32_t k = (i >> logS) & (R - 1);
+      const uint32_t pp = i >> (logS + LOGR);
+      const uint64_t pos = blk0 + i;
+      lds[lpad((k << logC) + (pp << logS) + jj)] = (pos < valid) ? src[pos] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
+  }
+
+  if (a.pre_lo) {  // h^pos on input position pos (coset shift)
+    const T *plo = (const T *)a.pre_lo, *phi = (const T *)a.pre_hi;
+    const uint64_t lomask = (1ull << a.lo_bits) - 1;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const uint64_t pos = pos0 + ((uint64_t)(t + m * RE) << logS);
+      x[m] = F::mulw(x[m], F::wmul(plo[pos & lomask], phi[pos >> a.lo_bits]));
+    }
+  }
+
+  if (a.flags & kTwiddle) {  // w_N^(k p s) on input row k
+    const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
+    const uint64_t ps = p << logS;
+    const uint64_t lomask = (1ull << a.lo_bits) - 1;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const uint64_t e = (uint64_t)(t + m * RE) * ps;
+      if (e) x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]));
+    }
+  }
+
+  // in-tile stage 0: stride R/E, one group per thread, no in-tile twiddle
+  dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0>(x, wE);
+  tile_stage<F, LOGR, LOGE, 1>(x, wE, lds, root, t, c, logC, a.root_shift);
+
+  // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
+  constexpr int EL = 1 << K::LOGLAST;
+  const uint32_t logNR = a.logN - LOGR;
+  const bool scale = (a.flags & kScale) != 0;
+  const T sw = (T)a.scale_w;
+#pragma unroll
+  for (int g = 0; g < E / EL; g++) {
+#pragma unroll
+    for (int kk = 0; kk < EL; kk++) {
+      const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
+      T v = x[g * EL + kk];
+      if (scale) v = F::mulw(v, sw);
+      const uint64_t oidx = ((uint64_t)krow << logNR) + q;
+      if (oidx < a.store_limit) dst[oidx] = v;
+    }
+  }
+}
+
+}  // namespace omr

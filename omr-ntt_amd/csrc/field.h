@@ -1,0 +1,159 @@
+// field.h -- prime-field arithmetic for the NTT engine (host and gfx950).
+//
+// The four fields of src/fields.rs:3-17.  Elements cross the C ABI and live
+// in HBM as canonical integers in [0, p).  Multipliers (twiddles, scales) are
+// kept in "multiplier form" W so that one product gives a canonical result:
+//   * 31/32-bit fields: Montgomery form w * 2^32 mod p, and
+//     mulw(x, W) = x * W * 2^-32 = x * w  (Montgomery reduction, R = 2^32).
+//   * Goldilocks: W = w, mulw is the 64x64 product with the special
+//     reduction 2^64 = 2^32 - 1, 2^96 = -1 (mod p).
+// CDNA4 has no 64-bit integer multiply; products are built from 32-bit
+// mul_lo / mul_hi (v_mul_lo_u32, v_mul_hi_u32, v_mad_u64_u32).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define OMR_HD __host__ __device__ __forceinline__
+#else
+#define OMR_HD inline
+#endif
+
+namespace omr {
+
+enum FieldId : int { kBabyBear = 0, kKoalaBear = 1, kGoldilocks = 2, kTeddyBear = 3 };
+
+constexpr uint32_t inv32(uint32_t p) {  // p^-1 mod 2^32 by Newton iteration
+  uint32_t x = p;
+  for (int i = 0; i < 5; i++) x *= 2u - p * x;
+  return x;
+}
+
+// 31/32-bit prime with Montgomery multipliers.  BIG = p > 2^31 (TeddyBear),
+// where a + b can carry out of 32 bits.
+template <uint32_t P_, uint32_t G_, int TWO_ADICITY_, int ID_>
+struct Mont32 {
+  using T = uint32_t;
+  static constexpr uint32_t P = P_;
+  static constexpr uint64_t G = G_;
+  static constexpr int TWO_ADICITY = TWO_ADICITY_;
+  static constexpr int ID = ID_;
+  static constexpr bool BIG = P_ > 0x80000000u;
+  static constexpr uint32_t PINV = inv32(P_);
+  static constexpr uint32_t R_MOD = (uint32_t)((1ull << 32) % P_);
+  static constexpr uint32_t R2 = (uint32_t)(((unsigned __int128)1 << 64) % P_);
+
+  OMR_HD static T add(T a, T b) {
+    if constexpr (!BIG) {
+      uint32_t s = a + b;
+      uint32_t t = s - P;
+      return t < s ? t : s;  // s >= P  <=>  s - P does not wrap
+    } else {
+      uint32_t nb = P - b;  // in (0, P]
+      uint32_t r = a - nb;
+      return a < nb ? r + P : r;
+    }
+  }
+  OMR_HD static T sub(T a, T b) {
+    uint32_t r = a - b;
+    return a < b ? r + P : r;
+  }
+  OMR_HD static T neg(T a) { return a ? P - a : 0; }
+  // Montgomery product a * b * 2^-32 mod p for a < p, b < 2^32.
+  OMR_HD static T mont(T a, T b) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t tlo = (uint32_t)t, thi = (uint32_t)(t >> 32);
+    uint32_t m = tlo * PINV;
+    uint32_t mh = (uint32_t)(((uint64_t)m * P) >> 32);
+    uint32_t r = thi - mh;
+    return thi < mh ? r + P : r;
+  }
+  OMR_HD static T mulw(T a, T w) { return mont(a, w); }
+  OMR_HD static T wmul(T w1, T w2) { return mont(w1, w2); }  // W x W -> W
+  OMR_HD static T to_w(T a) { return mont(a, R2); }
+  OMR_HD static T from_w(T w) { return mont(w, 1); }
+  OMR_HD static T mul(T a, T b) { return mont(mont(a, b), R2); }
+  OMR_HD static T one_w() { return R_MOD; }
+};
+
+using BabyBear = Mont32<2013265921u, 31, 27, kBabyBear>;
+using KoalaBear = Mont32<2130706433u, 3, 24, kKoalaBear>;
+using TeddyBear = Mont32<3221225473u, 5, 30, kTeddyBear>;
+
+struct Goldilocks {
+  using T = uint64_t;
+  static constexpr uint64_t P = 0xFFFFFFFF00000001ull;
+  static constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
+  static constexpr uint64_t G = 7;
+  static constexpr int TWO_ADICITY = 32;
+  static constexpr int ID = kGoldilocks;
+
+  OMR_HD static T add(T a, T b) {  // canonical in and out
+    uint64_t s = a + b;
+    bool carry = s < a;
+    return (carry || s >= P) ? s - P : s;  // on carry s - P == s + EPS mod 2^64
+  }
+  OMR_HD static T sub(T a, T b) {
+    uint64_t r = a - b;
+    return a < b ? r - EPS : r;  // r + P mod 2^64
+  }
+  OMR_HD static T neg(T a) { return a ? P - a : 0; }
+  // 128-bit product from four 32x32->64 partial products.
+  OMR_HD static void mul128(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+    uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    uint64_t p00 = (uint64_t)a0 * b0;
+    uint64_t p01 = (uint64_t)a0 * b1 + (p00 >> 32);
+    uint64_t p10 = (uint64_t)a1 * b0 + (uint32_t)p01;
+    uint64_t p11 = (uint64_t)a1 * b1 + (p01 >> 32) + (p10 >> 32);
+    lo = (p10 << 32) | (uint32_t)p00;
+    hi = p11;
+  }
+  // x = lo + hi * 2^64 -> canonical.
+  OMR_HD static T reduce128(uint64_t lo, uint64_t hi) {
+    uint64_t hh = hi >> 32, hl = hi & EPS;
+    uint64_t t0 = lo - hh;
+    if (lo < hh) t0 -= EPS;
+    uint64_t t1 = (hl << 32) - hl;
+    uint64_t r = t0 + t1;
+    if (r < t1) r += EPS;
+    return r >= P ? r - P : r;
+  }
+  OMR_HD static T mul(T a, T b) {
+    uint64_t lo, hi;
+    mul128(a, b, lo, hi);
+    return reduce128(lo, hi);
+  }
+  OMR_HD static T mulw(T a, T w) { return mul(a, w); }
+  OMR_HD static T wmul(T w1, T w2) { return mul(w1, w2); }
+  OMR_HD static T to_w(T a) { return a; }
+  OMR_HD static T from_w(T w) { return w; }
+  OMR_HD static T one_w() { return 1; }
+};
+
+// Host helpers on canonical values (uint64_t for every field).
+template <class F>
+inline uint64_t h_mul(uint64_t a, uint64_t b) {
+  return (uint64_t)(((unsigned __int128)a * b) % F::P);
+}
+template <class F>
+inline uint64_t h_pow(uint64_t a, uint64_t e) {
+  uint64_t r = 1;
+  while (e) {
+    if (e & 1) r = h_mul<F>(r, a);
+    a = h_mul<F>(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+template <class F>
+inline uint64_t h_inv(uint64_t a) {
+  return h_pow<F>(a, (uint64_t)F::P - 2);
+}
+// ark-ff get_root_of_unity convention: g^((p-1)/n) for n = 2^k.
+template <class F>
+inline uint64_t h_root_of_unity(uint64_t n) {
+  return h_pow<F>(F::G, ((uint64_t)F::P - 1) / n);
+}
+
+}  // namespace omr

@@ -1,0 +1,104 @@
+// internal.h -- shared host-side types of the engine library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/omrntt.h"
+
+namespace omr {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+// Device twiddle tables of one root of order 2^logN (multiplier form).
+struct TwTables {
+  void *lo = nullptr;    // root^e,               e < 2^lo_bits
+  void *hi = nullptr;    // root^(e 2^lo_bits),   e < 2^(logN - lo_bits)
+  void *root = nullptr;  // root^(m 2^(logN - log_rmax)), m < 2^log_rmax
+  uint32_t lo_bits = 0;
+  uint32_t log_rmax = 0;
+};
+
+// A transform schedule: pass radices 2^logR[i] (sum = logN), register radix
+// 2^logE (0 = per-pass default).
+struct Schedule {
+  std::vector<int> logR;
+  int logE = 0;
+};
+
+// Per-call options of run_transform.
+struct XformOpts {
+  bool scale = false;           // multiply every output by scale_w (last pass)
+  uint64_t scale_w = 0;         // multiplier form
+  bool zero_tail = false;       // input positions >= 2^log_valid read as zero (pass 0)
+  uint32_t log_valid = 0;
+  const void *pre_lo = nullptr;  // input-position multiplier tables (pass 0), or null
+  const void *pre_hi = nullptr;
+  uint64_t store_limit = ~0ull;  // only outputs with index < store_limit are stored
+};
+
+}  // namespace omr
+
+struct omr_ctx {
+  int device = 0;
+  std::mutex mu;
+  std::map<std::tuple<int, uint32_t, uint64_t>, omr::TwTables> tables;
+  omr::DevBuf scratch[3];
+  omr::DevBuf io[2];
+  omr::DevBuf part;
+};
+
+namespace omr {
+
+int field_bytes(int f);
+bool field_ok(int f);
+uint64_t fmul_host(int f, uint64_t a, uint64_t b);
+uint64_t fpow_host(int f, uint64_t a, uint64_t e);
+uint64_t finv_host(int f, uint64_t a);
+uint64_t to_w_host(int f, uint64_t a);
+uint64_t field_modulus(int f);
+uint64_t field_generator(int f);
+omr_status validate_domain(int f, uint64_t n, uint64_t omega);
+
+omr_status ensure_buf(DevBuf &b, size_t bytes);
+void free_buf(DevBuf &b);
+omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const TwTables **out);
+
+void default_schedule(int f, uint32_t logN, Schedule &s);
+omr_status variant_schedule(int f, int variant, uint32_t logN, Schedule &s);
+// Autotuned schedule for (field, logN) if the tuning table has one.
+bool tuned_schedule(int f, uint32_t logN, Schedule &s);
+
+omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
+                         uint64_t root, const Schedule &sch, uint64_t batch, uint64_t in_stride,
+                         uint64_t out_stride, const XformOpts &o, hipStream_t stream);
+omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void *d_out,
+                        uint32_t logN, uint64_t batch, uint64_t omega, hipStream_t stream);
+omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
+                        uint64_t omega, uint64_t out_len, int mode, int method, uint32_t log_n1,
+                        hipStream_t stream);
+
+// misc kernel launchers (misc_kernels.hip)
+omr_status launch_scale(int f, const void *in, void *out, uint64_t n, uint64_t w, int apply,
+                        hipStream_t s);
+omr_status launch_dft_full(int f, const void *in, void *out, uint64_t n, uint32_t logN,
+                           const TwTables *t, hipStream_t s);
+omr_status launch_dft_partial(int f, const void *in, void *part, uint64_t n, uint32_t out_len,
+                              uint32_t chunk, uint32_t nchunks, const TwTables *t, hipStream_t s);
+omr_status launch_reduce_rows(int f, const void *part, void *out, uint32_t nrows, uint64_t ld,
+                              uint64_t ncols, hipStream_t s);
+omr_status launch_td_recombine(int f, const void *Y, void *part, uint64_t out_len, uint64_t n2,
+                               uint32_t logN1, uint32_t splits, uint32_t logN, const TwTables *t,
+                               hipStream_t s);
+omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m, uint32_t nchunks,
+                       hipStream_t s);
+omr_status launch_field_op(int f, int op, const void *a, const void *b, void *out, uint64_t n,
+                           hipStream_t s);
+
+}  // namespace omr

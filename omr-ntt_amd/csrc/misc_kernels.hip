@@ -1,0 +1,128 @@
+// misc_kernels.hip -- field dispatch for the non-pass kernels of misc.h.
+#include "dispatch.h"
+#include "internal.h"
+#include "misc.h"
+
+namespace omr {
+
+#define OMR_FIELD_SWITCH(f, BODY)          \
+  switch (f) {                             \
+    case kBabyBear: {                      \
+      using F = BabyBear;                  \
+      BODY;                                \
+    } break;                               \
+    case kKoalaBear: {                     \
+      using F = KoalaBear;                 \
+      BODY;                                \
+    } break;                               \
+    case kGoldilocks: {                    \
+      using F = Goldilocks;                \
+      BODY;                                \
+    } break;                               \
+    case kTeddyBear: {                     \
+      using F = TeddyBear;                 \
+      BODY;                                \
+    } break;                               \
+    default:                               \
+      return OMR_E_FIELD;                  \
+  }
+
+static omr_status last_launch() { return hipGetLastError() == hipSuccess ? OMR_OK : OMR_E_DEVICE; }
+static unsigned blocks(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+static Tw2 tw2_of(const TwTables *t) { return Tw2{t->lo, t->hi, t->lo_bits}; }
+
+omr_status launch_scale(int f, const void *in, void *out, uint64_t n, uint64_t w, int apply,
+                        hipStream_t s) {
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_scale<F>, dim3(blocks(n, 256)), dim3(256), 0, s, (const T *)in, (T *)out, n,
+                       (T)w, apply);
+  });
+  return last_launch();
+}
+
+omr_status launch_dft_full(int f, const void *in, void *out, uint64_t n, uint32_t logN,
+                           const TwTables *t, hipStream_t s) {
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_dft_full<F>, dim3(blocks(n, 256)), dim3(256), 0, s, (const T *)in, (T *)out,
+                       n, logN, tw2_of(t));
+  });
+  return last_launch();
+}
+
+omr_status launch_dft_partial(int f, const void *in, void *part, uint64_t n, uint32_t out_len,
+                              uint32_t chunk, uint32_t nchunks, const TwTables *t, hipStream_t s) {
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_dft_partial_chunks<F>, dim3(nchunks), dim3(256), 0, s, (const T *)in,
+                       (T *)part, n, out_len, chunk, tw2_of(t));
+  });
+  return last_launch();
+}
+
+omr_status launch_reduce_rows(int f, const void *part, void *out, uint32_t nrows, uint64_t ld,
+                              uint64_t ncols, hipStream_t s) {
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_reduce_rows<F>, dim3(blocks(ncols, 256)), dim3(256), 0, s,
+                       (const T *)part, (T *)out, nrows, ld, ncols);
+  });
+  return last_launch();
+}
+
+omr_status launch_td_recombine(int f, const void *Y, void *part, uint64_t out_len, uint64_t n2,
+                               uint32_t logN1, uint32_t splits, uint32_t logN, const TwTables *t,
+                               hipStream_t s) {
+  const uint64_t rps = (n2 + splits - 1) / splits;
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    if (n2 >= 256) {
+      hipLaunchKernelGGL((k_td_recombine<F, 1>), dim3(blocks(out_len, 1), splits), dim3(256), 0, s,
+                         (const T *)Y, (T *)part, out_len, n2, logN1, rps, logN, tw2_of(t));
+    } else if (n2 >= 32) {
+      hipLaunchKernelGGL((k_td_recombine<F, 8>), dim3(blocks(out_len, 8), splits), dim3(256), 0, s,
+                         (const T *)Y, (T *)part, out_len, n2, logN1, rps, logN, tw2_of(t));
+    } else {
+      hipLaunchKernelGGL((k_td_recombine<F, 64>), dim3(blocks(out_len, 64), splits), dim3(256), 0, s,
+                         (const T *)Y, (T *)part, out_len, n2, logN1, rps, logN, tw2_of(t));
+    }
+  });
+  return last_launch();
+}
+
+omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m, uint32_t nchunks,
+                       hipStream_t s) {
+  const uint64_t nt = n / m;
+  const uint64_t tpc = (nt + nchunks - 1) / nchunks;
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_fold<F>, dim3(blocks(m, 256), nchunks), dim3(256), 0, s, (const T *)x,
+                       (T *)part, n, m, tpc);
+  });
+  return last_launch();
+}
+
+omr_status launch_field_op(int f, int op, const void *a, const void *b, void *out, uint64_t n,
+                           hipStream_t s) {
+  if (n == 0) return OMR_OK;
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_field_op<F>, dim3(blocks(n, 256)), dim3(256), 0, s, (const T *)a,
+                       (const T *)b, (T *)out, n, op);
+  });
+  return last_launch();
+}
+
+hipError_t launch_pass(int field, int logR, int logE, const PassArgs &a, dim3 grid, dim3 block,
+                       hipStream_t stream) {
+  switch (field) {
+    case kBabyBear: return launch_pass_babybear(logR, logE, a, grid, block, stream);
+    case kKoalaBear: return launch_pass_koalabear(logR, logE, a, grid, block, stream);
+    case kGoldilocks: return launch_pass_goldilocks(logR, logE, a, grid, block, stream);
+    case kTeddyBear: return launch_pass_teddybear(logR, logE, a, grid, block, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace omr

@@ -1,0 +1,331 @@
+"""omr_ntt -- Python host mirror of the omr-ntt operator API over the gfx950
+engine's C ABI (include/omrntt.h, libomrntt.so).
+
+Mirrors the reference crate's surface:
+  * fields (src/fields.rs:3-19): BabyBear, KoalaBear, Goldilocks, TeddyBear,
+    DefaultField = BabyBear;
+  * NttDomain(N) (src/encoder.rs:9-52): N, log_N, omega and the same panics
+    (here: OmrError) for non-power-of-two N and N beyond the two-adicity;
+  * NttEncoder.ntt(buf, domain) / .name() (src/encoder.rs:70-74) with the
+    twelve variant names of src/encoders/mod.rs:15-23;
+  * encoders.all(log_n) / encoders.all_partial(s) (src/encoders/mod.rs:32-54).
+
+buf is a numpy array of canonical values (host path, synchronous, in place)
+or a torch tensor on the GPU (device path, in place, on torch's current
+stream).  There is no CPU fallback: without libomrntt.so every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libomrntt.so")
+
+# ---- fields ----------------------------------------------------------------
+
+
+class Field:
+    def __init__(self, fid, name, p, g, two_adicity):
+        self.id, self.name, self.p, self.g, self.TWO_ADICITY = fid, name, p, g, two_adicity
+        self.bytes = 8 if p > 2**32 else 4
+        self.np_dtype = np.uint64 if self.bytes == 8 else np.uint32
+
+    def __repr__(self):
+        return self.name
+
+
+BabyBear = Field(0, "BabyBear", 2013265921, 31, 27)
+KoalaBear = Field(1, "KoalaBear", 2130706433, 3, 24)
+Goldilocks = Field(2, "Goldilocks", 18446744069414584321, 7, 32)
+TeddyBear = Field(3, "TeddyBear", 3221225473, 5, 30)
+DefaultField = BabyBear
+FIELDS = [BabyBear, KoalaBear, Goldilocks, TeddyBear]
+
+# ---- variants (src/encoders/mod.rs:15-23) -----------------------------------
+ARK_RADIX2, DFT, LAMBDA_BOWERS, LAMBDA_RADIX4, PLONKY3_LAYER_SPLIT, WINTERFELL_FOUR_STEP, \
+    TFHE_STOCKHAM_RADIX8, DFT_PARTIAL, TRANSFORM_DECOMPOSITION, TRANSFORM_DECOMPOSITION_V2 = range(10)
+
+STATUS = {0: "ok", 1: "E_NULL", 2: "E_SIZE", 3: "E_TWO_ADICITY", 4: "E_FIELD", 5: "E_VARIANT",
+          6: "E_RADIX", 7: "E_OMEGA", 8: "E_OUTLEN", 9: "E_ALLOC", 10: "E_DEVICE",
+          11: "E_SCHEDULE", 12: "E_ARG"}
+
+
+class OmrError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = lib().omr_status_str(status).decode()
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({msg})" if what else msg)
+
+
+_lib = None
+_ctx = None
+_VP = ctypes.c_void_p
+
+
+def lib():
+    """Loads libomrntt.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with python omr-ntt_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        i32, u64, u32 = ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32
+        P64, Pi = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)
+        sig = {
+            "omr_status_str": (ctypes.c_char_p, [i32]),
+            "omr_version": (ctypes.c_char_p, []),
+            "omr_ctx_create": (i32, [i32, ctypes.POINTER(_VP)]),
+            "omr_ctx_destroy": (i32, [_VP]),
+            "omr_ctx_synchronize": (i32, [_VP, _VP]),
+            "omr_field_info": (i32, [i32, P64, P64, Pi]),
+            "omr_field_bytes": (i32, [i32]),
+            "omr_root_of_unity": (i32, [i32, u64, P64]),
+            "omr_variant_name": (ctypes.c_char_p, [i32]),
+            "omr_variant_supports": (i32, [i32, u32]),
+            "omr_ntt": (i32, [_VP, i32, i32, _VP, u64, u64, u64]),
+            "omr_ntt_device": (i32, [_VP, i32, i32, _VP, _VP, u64, u64, u64, u64, _VP]),
+            "omr_intt_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, _VP]),
+            "omr_ntt_schedule_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, i32, Pi, i32, i32, _VP]),
+            "omr_default_schedule": (i32, [i32, u64, Pi, Pi]),
+            "omr_partial_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, i32, i32, u32, _VP]),
+            "omr_field_op_device": (i32, [i32, i32, _VP, _VP, _VP, u64, _VP]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def _check(status, what=""):
+    if status != 0:
+        raise OmrError(status, what)
+
+
+def context(device=0):
+    """Process-wide engine context on HIP device `device`."""
+    global _ctx
+    if _ctx is None:
+        c = _VP()
+        _check(lib().omr_ctx_create(device, ctypes.byref(c)), "omr_ctx_create")
+        _ctx = c
+    return _ctx
+
+
+def root_of_unity(field, n):
+    w = ctypes.c_uint64()
+    _check(lib().omr_root_of_unity(field.id, n, ctypes.byref(w)), f"root_of_unity(N={n})")
+    return w.value
+
+
+def default_schedule(field, n):
+    lr = (ctypes.c_int * 8)()
+    npass = ctypes.c_int()
+    _check(lib().omr_default_schedule(field.id, n, lr, ctypes.byref(npass)))
+    return [lr[i] for i in range(npass.value)]
+
+
+# ---- NttDomain (src/encoder.rs:9-52) ---------------------------------------
+
+
+class NttDomain:
+    """Reusable NTT parameters: N, log_N and the primitive N-th root omega
+    (F::get_root_of_unity(N)).  The reference also materialises twiddles /
+    bitrev_twiddles on the host; the engine derives its own device tables from
+    omega, so they are computed only on request."""
+
+    def __init__(self, N, field=DefaultField, omega=None):
+        if N <= 0 or N & (N - 1):
+            raise OmrError(2, f"NTT size N={N} must be a power of 2")
+        log_N = N.bit_length() - 1
+        if log_N > field.TWO_ADICITY:
+            raise OmrError(3, f"N={N} requires log2(N)={log_N} <= TWO_ADICITY={field.TWO_ADICITY}")
+        self.field, self.N, self.log_N = field, N, log_N
+        self.omega = root_of_unity(field, N) if omega is None else int(omega)
+
+    @property
+    def twiddles(self):
+        p, w = self.field.p, self.omega
+        out, cur = [], 1
+        for _ in range(self.N):
+            out.append(cur)
+            cur = cur * w % p
+        return np.array(out, dtype=np.uint64)
+
+
+# ---- device-pointer helpers --------------------------------------------------
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _stream_ptr():
+    import torch
+    return _VP(torch.cuda.current_stream().cuda_stream)
+
+
+def _dptr(t):
+    return _VP(t.data_ptr())
+
+
+def _check_dev_tensor(t, field, n_elems=None):
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("device buffers must be contiguous GPU tensors")
+    if t.element_size() != field.bytes:
+        raise ValueError(f"{field} elements are {field.bytes} bytes, tensor has {t.element_size()}")
+    if n_elems is not None and t.numel() < n_elems:
+        raise ValueError("tensor too small")
+
+
+# ---- device entry points -----------------------------------------------------
+
+
+def ntt_device(x, out, field, variant=ARK_RADIX2, omega=None, s=0, batch=1, n=None):
+    n = n or x.numel() // batch
+    omega = root_of_unity(field, n) if omega is None else omega
+    _check_dev_tensor(x, field, n * batch)
+    _check_dev_tensor(out, field)
+    _check(lib().omr_ntt_device(context(), field.id, variant, _dptr(x), _dptr(out), n, batch, omega, s,
+                                _stream_ptr()), "omr_ntt_device")
+    return out
+
+
+def intt_device(x, out, field, omega=None, batch=1, n=None):
+    n = n or x.numel() // batch
+    omega = root_of_unity(field, n) if omega is None else omega
+    _check_dev_tensor(x, field, n * batch)
+    _check_dev_tensor(out, field, n * batch)
+    _check(lib().omr_intt_device(context(), field.id, _dptr(x), _dptr(out), n, batch, omega,
+                                 _stream_ptr()), "omr_intt_device")
+    return out
+
+
+def ntt_schedule_device(x, out, field, log_radix, log_e=0, inverse=False, omega=None, batch=1, n=None):
+    n = n or x.numel() // batch
+    omega = root_of_unity(field, n) if omega is None else omega
+    _check_dev_tensor(x, field, n * batch)
+    _check_dev_tensor(out, field, n * batch)
+    lr = (ctypes.c_int * len(log_radix))(*log_radix)
+    _check(lib().omr_ntt_schedule_device(context(), field.id, _dptr(x), _dptr(out), n, batch, omega,
+                                         1 if inverse else 0, lr, len(log_radix), log_e,
+                                         _stream_ptr()), "omr_ntt_schedule_device")
+    return out
+
+
+PREFIX, STRIDE = 0, 1
+
+
+def partial_device(x, out, field, out_len, mode=PREFIX, method=0, log_n1=0, omega=None):
+    n = x.numel()
+    omega = root_of_unity(field, n) if omega is None else omega
+    _check_dev_tensor(x, field, n)
+    _check_dev_tensor(out, field, out_len)
+    _check(lib().omr_partial_device(context(), field.id, _dptr(x), _dptr(out), n, omega, out_len,
+                                    mode, method, log_n1, _stream_ptr()), "omr_partial_device")
+    return out
+
+
+def field_op_device(field, op, a, b, out):
+    _check(lib().omr_field_op_device(field.id, op, _dptr(a), _dptr(b), _dptr(out), a.numel(),
+                                     _stream_ptr()), "omr_field_op_device")
+    return out
+
+
+def synchronize():
+    _check(lib().omr_ctx_synchronize(context(), _stream_ptr()))
+
+
+# ---- tensors <-> numpy -----------------------------------------------------
+
+
+def torch_dtype(field):
+    import torch
+    return torch.int64 if field.bytes == 8 else torch.int32
+
+
+def to_device(arr, field):
+    """numpy canonical values -> contiguous GPU tensor (bit-identical)."""
+    import torch
+    a = np.ascontiguousarray(np.asarray(arr).astype(field.np_dtype))
+    view = a.view(np.int64 if field.bytes == 8 else np.int32)
+    return torch.from_numpy(view.copy()).cuda()
+
+
+def from_device(t, field):
+    a = t.detach().cpu().numpy()
+    return a.view(np.uint64 if field.bytes == 8 else np.uint32).astype(np.uint64)
+
+
+def empty_device(n, field):
+    import torch
+    return torch.empty(n, dtype=torch_dtype(field), device="cuda")
+
+
+# ---- the operator API (src/encoder.rs:70-83) ---------------------------------
+
+
+class NttEncoder:
+    variant = None
+    s = 0
+
+    def ntt(self, buf, domain):
+        field = domain.field
+        if _is_torch(buf):  # device-resident, in place
+            out_len = buf.numel() if self.variant < DFT_PARTIAL else min(2 * self.s, domain.N)
+            if self.variant >= DFT_PARTIAL:
+                tmp = empty_device(max(out_len, 1), field)
+                ntt_device(buf, tmp, field, self.variant, domain.omega, self.s, n=domain.N)
+                buf[:out_len].copy_(tmp[:out_len])
+            else:
+                ntt_device(buf, buf, field, self.variant, domain.omega, self.s, n=domain.N)
+            return
+        if not isinstance(buf, np.ndarray) or buf.dtype != field.np_dtype or not buf.flags["C_CONTIGUOUS"]:
+            raise TypeError(f"buf must be a contiguous numpy {field.np_dtype.__name__} array")
+        if buf.size != domain.N:
+            raise OmrError(12, f"buf.len()={buf.size} != N={domain.N}")
+        _check(lib().omr_ntt(context(), field.id, self.variant, buf.ctypes.data_as(_VP), domain.N,
+                             domain.omega, self.s), self.name())
+
+    def name(self):
+        return lib().omr_variant_name(self.variant).decode()
+
+
+def _full(variant_id, cls_name):
+    return type(cls_name, (NttEncoder,), {"variant": variant_id})
+
+
+def _partial(variant_id, cls_name):
+    def __init__(self, s):
+        self.s = s
+    return type(cls_name, (NttEncoder,), {"variant": variant_id, "__init__": __init__})
+
+
+ArkRadix2 = _full(ARK_RADIX2, "ArkRadix2")
+Dft = _full(DFT, "Dft")
+LambdaBowers = _full(LAMBDA_BOWERS, "LambdaBowers")
+LambdaRadix4 = _full(LAMBDA_RADIX4, "LambdaRadix4")
+Plonky3Radix2LayerSplit = _full(PLONKY3_LAYER_SPLIT, "Plonky3Radix2LayerSplit")
+WinterfellFourStep = _full(WINTERFELL_FOUR_STEP, "WinterfellFourStep")
+TfheStockhamRadix8 = _full(TFHE_STOCKHAM_RADIX8, "TfheStockhamRadix8")
+DftPartial = _partial(DFT_PARTIAL, "DftPartial")
+TransformDecomposition = _partial(TRANSFORM_DECOMPOSITION, "TransformDecomposition")
+TransformDecompositionV2 = _partial(TRANSFORM_DECOMPOSITION_V2, "TransformDecompositionV2")
+
+
+class encoders:
+    """Registry (src/encoders/mod.rs:32-54)."""
+
+    @staticmethod
+    def all(log_n):
+        v = [ArkRadix2(), LambdaBowers(), WinterfellFourStep(), Plonky3Radix2LayerSplit()]
+        if log_n % 2 == 0:
+            v.append(LambdaRadix4())
+        if log_n % 3 == 0:
+            v.append(TfheStockhamRadix8())
+        return v
+
+    @staticmethod
+    def all_partial(s):
+        return [DftPartial(s), TransformDecomposition(s), TransformDecompositionV2(s)]

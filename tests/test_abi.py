@@ -1,0 +1,93 @@
+"""CPU-side checks of the C ABI: the library loads, exports every function
+include/omrntt.h declares, and rejects bad arguments before any GPU work
+(the reference's panics, src/encoder.rs:24-32, lambdaworks_radix4.rs:17-21,
+tfhe_stockham_radix8.rs:17-21, become status codes)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import omr_ntt as E
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    inc = os.path.join(ROOT, "include")
+    for fn in sorted(os.listdir(inc)):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(inc, fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names += re.findall(r"\b(omr_[a-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = E.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_field_info_and_roots():
+    for F in E.FIELDS:
+        p, g, s = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        assert E.lib().omr_field_info(F.id, ctypes.byref(p), ctypes.byref(g), ctypes.byref(s)) == 0
+        assert (p.value, g.value, s.value) == (F.p, F.g, F.TWO_ADICITY)
+        for k in range(0, F.TWO_ADICITY + 1, 3):
+            assert E.root_of_unity(F, 1 << k) == pow(F.g, (F.p - 1) >> k, F.p)
+    assert E.lib().omr_field_info(7, None, None, None) == 4
+
+
+def test_domain_errors_mirror_reference_panics():
+    with pytest.raises(E.OmrError) as e:
+        E.NttDomain(12, E.BabyBear)
+    assert e.value.status == 2
+    with pytest.raises(E.OmrError) as e:
+        E.NttDomain(1 << 28, E.BabyBear)
+    assert e.value.status == 3
+    with pytest.raises(E.OmrError):
+        E.root_of_unity(E.KoalaBear, 1 << 25)
+
+
+def _call_ntt(field, variant, n, omega, s=0, buf=True):
+    b = np.zeros(max(n, 1), dtype=np.uint64)
+    ptr = b.ctypes.data_as(ctypes.c_void_p) if buf else None
+    return E.lib().omr_ntt(None, field, variant, ptr, n, omega, s)
+
+
+def test_validation_before_device_work():
+    L = E.lib()
+    gl = E.Goldilocks
+    w = E.root_of_unity(gl, 1024)
+    assert _call_ntt(2, 0, 1000, w) == 2          # not a power of two
+    assert _call_ntt(0, 0, 1 << 28, 1) == 3       # beyond BabyBear two-adicity
+    assert _call_ntt(9, 0, 1024, w) == 4          # unknown field
+    assert _call_ntt(2, 42, 1024, w) == 5         # unknown variant
+    assert _call_ntt(2, 3, 512, E.root_of_unity(gl, 512)) == 6   # radix-4 needs even log N
+    assert _call_ntt(2, 6, 1024, w) == 6          # radix-8 needs log N % 3 == 0
+    assert _call_ntt(2, 0, 1024, pow(w, 2, gl.p)) == 7   # omega of order N/2
+    assert _call_ntt(2, 0, 1024, gl.p + 5) == 7          # omega out of range
+    assert _call_ntt(2, 0, 1024, w) == 1                 # valid args, null context
+    lr = (ctypes.c_int * 2)(10, 10)
+    assert L.omr_ntt_schedule_device(None, 2, None, None, 1024, 1, w, 0, lr, 2, 0, None) == 11
+    lr = (ctypes.c_int * 2)(13, 0)
+    assert L.omr_ntt_schedule_device(None, 2, None, None, 1 << 13, 1, E.root_of_unity(gl, 1 << 13), 0, lr, 1, 0, None) == 11
+    assert L.omr_partial_device(None, 2, None, None, 1024, w, 2048, 0, 0, 0, None) == 8
+    assert L.omr_partial_device(None, 2, None, None, 1024, w, 24, 1, 0, 0, None) == 8
+    assert L.omr_variant_name(99) is None
+    assert [L.omr_variant_name(i).decode() for i in range(10)] == [
+        "ArkRadix2", "Dft", "LambdaBowers", "LambdaRadix4", "Plonky3Radix2LayerSplit",
+        "WinterfellFourStep", "TfheStockhamRadix8", "DftPartial", "TransformDecomposition",
+        "TransformDecompositionV2"]
+
+
+def test_default_schedules():
+    assert E.default_schedule(E.Goldilocks, 1 << 24) == [8, 8, 8]
+    assert E.default_schedule(E.BabyBear, 1 << 16) == [8, 8]
+    assert sum(E.default_schedule(E.Goldilocks, 1 << 20)) == 20
+    assert E.default_schedule(E.Goldilocks, 1 << 12) == [12]

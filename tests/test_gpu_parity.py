@@ -1,0 +1,237 @@
+"""GPU parity: the HIP engine through the C ABI against the CPU oracle and
+the pure-Python golden vectors.  Mirrors the reference's relational matrix
+(src/lib.rs:24-180) for all four fields and extends it to the config sizes."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import omr_ntt as E  # noqa: E402
+import oracle as O  # noqa: E402
+
+from _gpu import inputs, gpu_ntt  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+POWERS_OF_TWO = [1 << k for k in range(1, 17)]
+EVEN_POWERS_OF_TWO = [1 << k for k in range(2, 17, 2)]
+POWERS_OF_TWO_DIV_BY_3 = [8, 64, 512, 4096, 32768]
+FIELD_IDS = [f.name for f in E.FIELDS]
+
+
+def field(name):
+    return {f.name: f for f in E.FIELDS}[name]
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as fh:
+        return json.load(fh)
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_device_field_ops_golden(fname):
+    F = field(fname)
+    g = load("fieldops.json")[fname.lower()]
+    a = E.to_device(np.array(g["a"], dtype=np.uint64), F)
+    b = E.to_device(np.array(g["b"], dtype=np.uint64), F)
+    out = E.empty_device(a.numel(), F)
+    for op, key in ((0, "add"), (1, "sub"), (2, "mul"), (4, "mul")):
+        E.field_op_device(F, op, a, b, out)
+        assert [int(v) for v in E.from_device(out, F)] == g[key], (fname, key, op)
+    E.field_op_device(F, 3, a, b, out)
+    assert [int(v) for v in E.from_device(out, F)] == g["a"]
+    # random bulk vs oracle
+    x, y = inputs(F, 1 << 16, 1), inputs(F, 1 << 16, 2)
+    a, b = E.to_device(x, F), E.to_device(y, F)
+    out = E.empty_device(x.size, F)
+    for op in (0, 1, 2):
+        E.field_op_device(F, op, a, b, out)
+        assert np.array_equal(E.from_device(out, F), O.vec_op(F.id, op, x, y))
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_dft_golden_vectors(fname):
+    F = field(fname)
+    for case in load("dft.json")[fname.lower()]:
+        x = np.array(case["x"], dtype=np.uint64)
+        for variant in (E.ARK_RADIX2, E.DFT):
+            got = gpu_ntt(F, x, variant)
+            assert [int(v) for v in got] == case["X"], (fname, case["n"], variant)
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+@pytest.mark.parametrize("enc", ["ArkRadix2", "LambdaBowers", "WinterfellFourStep", "Plonky3Radix2LayerSplit"])
+def test_full_encoders_agree_with_oracle(fname, enc):  # src/lib.rs:114-127
+    F = field(fname)
+    for n in POWERS_OF_TWO:
+        x = inputs(F, n, 1000 + n)
+        exp = O.encoder("ArkRadix2", F.id, x)
+        dom = E.NttDomain(n, F)
+        buf = x.astype(F.np_dtype)
+        getattr(E, enc)().ntt(buf, dom)  # host path (the trait drop-in)
+        assert np.array_equal(buf.astype(np.uint64), exp), (enc, n)
+        t = E.to_device(x, F)
+        getattr(E, enc)().ntt(t, dom)  # device path
+        assert np.array_equal(E.from_device(t, F), exp), (enc, n)
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_radix4_stockham8_and_constraints(fname):  # src/lib.rs:130-139
+    F = field(fname)
+    for n in EVEN_POWERS_OF_TWO:
+        x = inputs(F, n, n)
+        assert np.array_equal(gpu_ntt(F, x, E.LAMBDA_RADIX4), O.ntt(F.id, x)), n
+    for n in POWERS_OF_TWO_DIV_BY_3:
+        x = inputs(F, n, n)
+        assert np.array_equal(gpu_ntt(F, x, E.TFHE_STOCKHAM_RADIX8), O.ntt(F.id, x)), n
+    with pytest.raises(E.OmrError):
+        E.LambdaRadix4().ntt(inputs(F, 8, 1).astype(F.np_dtype), E.NttDomain(8, F))
+    with pytest.raises(E.OmrError):
+        E.TfheStockhamRadix8().ntt(inputs(F, 16, 1).astype(F.np_dtype), E.NttDomain(16, F))
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_dft_variant_agrees(fname):  # src/lib.rs:85-88
+    F = field(fname)
+    for n in (64, 128, 256, 4096):
+        x = inputs(F, n, n)
+        assert np.array_equal(gpu_ntt(F, x, E.DFT), O.ntt(F.id, x))
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_partial_encoders_prefix(fname):  # src/lib.rs:141-165
+    F = field(fname)
+    cases = [("DftPartial", [64, 128, 256]), ("TransformDecomposition", POWERS_OF_TWO),
+             ("TransformDecompositionV2", [1024, 4096, 32768, 65536])]
+    for enc, sizes in cases:
+        for s in (2, 4, 8, 16, 50):
+            for n in sizes:
+                x = inputs(F, n, s * 7 + n)
+                m = min(2 * s, n)
+                exp = O.ntt(F.id, x)
+                buf = x.astype(F.np_dtype)
+                getattr(E, enc)(s).ntt(buf, E.NttDomain(n, F))
+                got = buf.astype(np.uint64)
+                assert np.array_equal(got[:m], exp[:m]), (enc, s, n)
+                assert np.array_equal(got[m:], x[m:]), (enc, s, n)  # tail keeps the input
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_inverse_round_trip(fname):
+    F = field(fname)
+    for n in [2, 16, 1 << 10, 1 << 13, 1 << 16, 1 << 18]:
+        x = inputs(F, n, 77 + n)
+        t = E.to_device(x, F)
+        E.ntt_device(t, t, F)
+        X = E.from_device(t, F)
+        assert np.array_equal(X, O.ntt(F.id, x))
+        E.intt_device(t, t, F)
+        assert np.array_equal(E.from_device(t, F), x), n
+        if n <= 1 << 13:
+            assert np.array_equal(O.intt(F.id, X), x)
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_explicit_schedules_agree(fname):
+    F = field(fname)
+    maxlr = 12 if F.bytes == 8 else 13
+    n = 1 << 16
+    x = inputs(F, n, 5)
+    exp = O.ntt(F.id, x)
+    t_in = E.to_device(x, F)
+    scheds = [[16], [8, 8], [4, 12], [12, 4], [6, 5, 5], [2, 3, 4, 7], [1, 15], [3] * 5 + [1], [2] * 8]
+    for sched in scheds:
+        if any(r > maxlr for r in sched):
+            continue
+        for log_e in (0, 2, 3, 4):
+            out = E.empty_device(n, F)
+            E.ntt_schedule_device(t_in, out, F, sched, log_e=log_e)
+            assert np.array_equal(E.from_device(out, F), exp), (sched, log_e)
+            back = E.empty_device(n, F)
+            E.ntt_schedule_device(out, back, F, sched, log_e=log_e, inverse=True)
+            assert np.array_equal(E.from_device(back, F), x), (sched, log_e)
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_batched_same_size(fname):
+    F = field(fname)
+    n, batch = 1 << 12, 37
+    x = inputs(F, n * batch, 11)
+    t = E.to_device(x, F)
+    out = E.empty_device(n * batch, F)
+    E.ntt_device(t, out, F, batch=batch, n=n)
+    got = E.from_device(out, F).reshape(batch, n)
+    for b in (0, 1, 17, 36):
+        assert np.array_equal(got[b], O.ntt(F.id, x[b * n:(b + 1) * n]))
+
+
+def test_custom_omega_is_honoured():
+    # any primitive N-th root (not only the ark-convention one) is accepted
+    F = E.Goldilocks
+    n = 1024
+    w = E.root_of_unity(F, n)
+    w3 = pow(w, 3, F.p)  # also primitive (3 is odd)
+    x = inputs(F, n, 3)
+    t = E.to_device(x, F)
+    E.ntt_device(t, t, F, omega=w3)
+    assert np.array_equal(E.from_device(t, F), O.ntt(F.id, x, omega=w3))
+    with pytest.raises(E.OmrError):
+        E.ntt_device(t, t, F, omega=pow(w, 2, F.p))  # order N/2: not primitive
+
+
+@pytest.mark.parametrize("logn", [20, 22, 24])
+def test_goldilocks_large_forward_inverse(logn):  # config 2
+    F = E.Goldilocks
+    n = 1 << logn
+    x = inputs(F, n, 42)
+    t = E.to_device(x, F)
+    out = E.empty_device(n, F)
+    E.ntt_device(t, out, F)
+    X = E.from_device(out, F)
+    assert np.array_equal(X, O.ntt(F.id, x))
+    back = E.empty_device(n, F)
+    E.intt_device(out, back, F)
+    assert torch.equal(back, t)
+    # four-step 2^12 x 2^12 at 2^24 and the radix-8 Stockham agree too
+    if logn == 24:
+        for sched, le in (([12, 12], 0), ([8, 8, 8], 0), ([9, 9, 6], 3), ([6, 6, 6, 6], 0)):
+            o2 = E.empty_device(n, F)
+            E.ntt_schedule_device(t, o2, F, sched, log_e=le)
+            assert torch.equal(o2, out), sched
+
+
+@pytest.mark.parametrize("fname", ["BabyBear", "Goldilocks"])
+def test_partial_large_prefix_and_stride(fname):  # config 3
+    F = field(fname)
+    n = 1 << 22
+    x = inputs(F, n, 42)
+    X = O.ntt(F.id, x)
+    t = E.to_device(x, F)
+    for logs in (8, 12, 16, 20):
+        m = 1 << logs
+        out = E.empty_device(m, F)
+        for method in (0, 2, 3):
+            if method == 2 and m > (1 << 14):
+                continue
+            E.partial_device(t, out, F, m, mode=E.PREFIX, method=method)
+            assert np.array_equal(E.from_device(out, F), X[:m]), (logs, method)
+        E.partial_device(t, out, F, m, mode=E.STRIDE)
+        assert np.array_equal(E.from_device(out, F), X[:: n // m]), logs
+    # reference quirk sizes (src/lib.rs:167-180): s in {1, 3, 101}
+    for s in (1, 3, 101):
+        m = 2 * s
+        out = E.empty_device(m, F)
+        for method in (1, 2, 3):
+            E.partial_device(t, out, F, m, method=method)
+            assert np.array_equal(E.from_device(out, F), X[:m]), (s, method)
+
+
+def test_encoder_registry_names():
+    names = [e.name() for e in E.encoders.all(12)]
+    assert names == ["ArkRadix2", "LambdaBowers", "WinterfellFourStep", "Plonky3Radix2LayerSplit",
+                     "LambdaRadix4", "TfheStockhamRadix8"]
+    assert [e.name() for e in E.encoders.all_partial(50)] == [
+        "DftPartial", "TransformDecomposition", "TransformDecompositionV2"]
