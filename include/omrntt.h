@@ -136,6 +136,24 @@ omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_in, void *d
                               uint64_t omega, uint64_t out_len, int mode, int method,
                               uint32_t log_n1, void *stream);
 
+/* Coset low-degree extension of `ncols` columns (the encoder path; the
+ * reference's decode.rs is a stub, so this is the engine's own closed form):
+ *   evals[c][i] = sum_{j<n} coeffs[c][j] (shift * w_m^i)^j,  i < m = n * blowup,
+ * w_m the ark-convention root of unity of order m; shift = 0 selects the
+ * field generator g (the coset gH).  Column c of coeffs is at d_coeffs + c*n,
+ * of evals at d_evals + c*m.  n and blowup powers of two. */
+omr_status omr_coset_lde_device(omr_ctx *ctx, int field, const void *d_coeffs, void *d_evals,
+                                uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t shift,
+                                void *stream);
+
+/* Decode of omr_coset_lde_device: coeffs[c][j] = shift^-j m^-1 sum_i
+ * evals[c][i] w_m^(-ij) for j < n.  If d_nonzero (device uint32) is given it
+ * is set to the number of nonzero coefficients at j >= n over all columns
+ * (0 for codewords). */
+omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const void *d_evals, void *d_coeffs,
+                                   uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t shift,
+                                   uint32_t *d_nonzero, void *stream);
+
 /* Elementwise device field arithmetic on canonical arrays (test hook):
  * op 0 add, 1 sub, 2 mul, 3 multiplier-form round trip of a, 4 a * b via
  * multiplier form. */

@@ -15,7 +15,6 @@
 #include "../../include/omrntt.h"
 #include "dispatch.h"
 #include "internal.h"
-#include "misc.h"
 
 using namespace omr;
 
@@ -355,9 +354,14 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.store_limit = (i == np - 1) ? o.store_limit : ~0ull;
     a.flags = 0;
     if (i > 0) a.flags |= kTwiddle;
-    if (i == np - 1 && o.scale) {
-      a.flags |= kScale;
-      a.scale_w = o.scale_w;
+    if (i == np - 1) {
+      if (o.scale) {
+        a.flags |= kScale;
+        a.scale_w = o.scale_w;
+      }
+      a.post_lo = o.post_lo;
+      a.post_hi = o.post_hi;
+      a.nonzero_count = o.nonzero_count;
     }
     if (i == 0) {
       if (o.zero_tail) {

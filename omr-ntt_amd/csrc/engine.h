@@ -27,6 +27,9 @@ struct PassArgs {
   const void *root;    // w_Rmax^m, m < Rmax                        (multiplier form)
   const void *pre_lo;  // optional input-position multiplier h^i:   h^(i mod 2^lo_bits)
   const void *pre_hi;  //                                            h^((i >> lo_bits) 2^lo_bits)
+  const void *post_lo;  // optional output-position multiplier, same two-level layout
+  const void *post_hi;
+  uint32_t *nonzero_count;  // if set: outputs at index >= store_limit that are nonzero are counted
   uint64_t vec_stride_in;   // elements between consecutive input vectors (blockIdx.y)
   uint64_t vec_stride_out;  // elements between consecutive output vectors
   uint64_t scale_w;         // multiplier applied to every output when kScale
@@ -240,7 +243,12 @@ k_stockham_pass(PassArgs a) {
       T v = x[g * EL + kk];
       if (scale) v = F::mulw(v, sw);
       const uint64_t oidx = ((uint64_t)krow << logNR) + q;
+      if (a.post_lo) {
+        const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
+        v = F::mulw(v, F::wmul(plo[oidx & ((1ull << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
+      }
       if (oidx < a.store_limit) dst[oidx] = v;
+      else if (a.nonzero_count && v != 0) atomicAdd(a.nonzero_count, 1u);
     }
   }
 }

@@ -41,6 +41,9 @@ struct XformOpts {
   const void *pre_lo = nullptr;  // input-position multiplier tables (pass 0), or null
   const void *pre_hi = nullptr;
   uint64_t store_limit = ~0ull;  // only outputs with index < store_limit are stored
+  const void *post_lo = nullptr;  // output-position multiplier tables (last pass), or null
+  const void *post_hi = nullptr;
+  uint32_t *nonzero_count = nullptr;  // counts nonzero outputs at index >= store_limit
 };
 
 }  // namespace omr
@@ -98,6 +101,7 @@ omr_status launch_td_recombine(int f, const void *Y, void *part, uint64_t out_le
                                hipStream_t s);
 omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m, uint32_t nchunks,
                        hipStream_t s);
+omr_status launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 omr_status launch_field_op(int f, int op, const void *a, const void *b, void *out, uint64_t n,
                            hipStream_t s);
 

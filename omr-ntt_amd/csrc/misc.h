@@ -162,4 +162,9 @@ __global__ void __launch_bounds__(256) k_scale(const typename F::T *__restrict__
   out[i] = apply ? F::mulw(x[i], w) : x[i];
 }
 
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 }  // namespace omr

@@ -103,6 +103,12 @@ omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m,
   return last_launch();
 }
 
+omr_status launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return OMR_OK;
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n, 256)), dim3(256), 0, s, p, n, v);
+  return last_launch();
+}
+
 omr_status launch_field_op(int f, int op, const void *a, const void *b, void *out, uint64_t n,
                            hipStream_t s) {
   if (n == 0) return OMR_OK;

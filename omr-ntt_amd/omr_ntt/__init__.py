@@ -90,6 +90,8 @@ def lib():
             "omr_default_schedule": (i32, [i32, u64, Pi, Pi]),
             "omr_partial_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, i32, i32, u32, _VP]),
             "omr_field_op_device": (i32, [i32, i32, _VP, _VP, _VP, u64, _VP]),
+            "omr_coset_lde_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, _VP]),
+            "omr_coset_decode_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, _VP, _VP]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -225,6 +227,26 @@ def partial_device(x, out, field, out_len, mode=PREFIX, method=0, log_n1=0, omeg
     _check(lib().omr_partial_device(context(), field.id, _dptr(x), _dptr(out), n, omega, out_len,
                                     mode, method, log_n1, _stream_ptr()), "omr_partial_device")
     return out
+
+
+def coset_lde_device(coeffs, evals, field, n, ncols=1, blowup=4, shift=0):
+    """evals[c][i] = sum_j coeffs[c][j] (shift w_m^i)^j, m = n * blowup (shift 0 = g)."""
+    _check_dev_tensor(coeffs, field, n * ncols)
+    _check_dev_tensor(evals, field, n * blowup * ncols)
+    _check(lib().omr_coset_lde_device(context(), field.id, _dptr(coeffs), _dptr(evals), n, ncols,
+                                      blowup, shift, _stream_ptr()), "omr_coset_lde_device")
+    return evals
+
+
+def coset_decode_device(evals, coeffs, field, n, ncols=1, blowup=4, shift=0, nonzero=None):
+    """Inverse of coset_lde_device; nonzero (int32 GPU tensor of 1) receives the
+    count of nonzero coefficients at j >= n."""
+    _check_dev_tensor(evals, field, n * blowup * ncols)
+    _check_dev_tensor(coeffs, field, n * ncols)
+    nz = _dptr(nonzero) if nonzero is not None else None
+    _check(lib().omr_coset_decode_device(context(), field.id, _dptr(evals), _dptr(coeffs), n, ncols,
+                                         blowup, shift, nz, _stream_ptr()), "omr_coset_decode_device")
+    return coeffs
 
 
 def field_op_device(field, op, a, b, out):

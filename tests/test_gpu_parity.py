@@ -235,3 +235,52 @@ def test_encoder_registry_names():
                      "LambdaRadix4", "TfheStockhamRadix8"]
     assert [e.name() for e in E.encoders.all_partial(50)] == [
         "DftPartial", "TransformDecomposition", "TransformDecompositionV2"]
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_coset_lde_golden_and_decode(fname):  # config 5 semantics at golden sizes
+    F = field(fname)
+    for case in load("lde.json")[fname.lower()]:
+        n, blow = case["n"], case["blowup"]
+        c = E.to_device(np.array(case["coeffs"], dtype=np.uint64), F)
+        ev = E.empty_device(n * blow, F)
+        E.coset_lde_device(c, ev, F, n, 1, blow, case["shift"])
+        assert [int(v) for v in E.from_device(ev, F)] == case["evals"]
+        back = E.empty_device(n, F)
+        nz = torch.zeros(1, dtype=torch.int32, device="cuda")
+        E.coset_decode_device(ev, back, F, n, 1, blow, case["shift"], nonzero=nz)
+        assert [int(v) for v in E.from_device(back, F)] == case["coeffs"]
+        assert int(nz.item()) == 0
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_coset_lde_columns(fname):
+    F = field(fname)
+    n, ncols, blow = 1 << 12, 8, 4
+    x = inputs(F, n * ncols, 21)
+    c = E.to_device(x, F)
+    ev = E.empty_device(n * blow * ncols, F)
+    E.coset_lde_device(c, ev, F, n, ncols, blow)
+    got = E.from_device(ev, F).reshape(ncols, n * blow)
+    for col in (0, 3, 7):
+        assert np.array_equal(got[col], O.coset_lde(F.id, x[col * n:(col + 1) * n], blow))
+    back = E.empty_device(n * ncols, F)
+    nz = torch.zeros(1, dtype=torch.int32, device="cuda")
+    E.coset_decode_device(ev, back, F, n, ncols, blow, nonzero=nz)
+    assert torch.equal(back, c) and int(nz.item()) == 0
+    # a corrupted evaluation is detected (its decode has nonzero high coefficients)
+    bad = ev.clone()
+    bad[5] = (bad[5] + 1) % F.p if F.bytes == 4 else bad[5] ^ 1
+    E.coset_decode_device(bad, back, F, n, ncols, blow, nonzero=nz)
+    assert int(nz.item()) > 0
+
+
+def test_cpp_reference_test_suite():
+    """tests/cpp/test_encoders.cpp: the reference's src/lib.rs tests in C++
+    through include/omrntt.hpp (built by __graft_entry__.build)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "test_encoders")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/test_encoders not built (run __graft_entry__.build())")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
