@@ -154,6 +154,40 @@ omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const void *d_evals,
                                    uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t shift,
                                    uint32_t *d_nonzero, void *stream);
 
+/* ---- batches of vectors of uneven power-of-two sizes --------------------- */
+/* One vector of a batch: element offsets into the batch's input and output
+ * buffers, size 2^log_n (4 <= log_n <= two-adicity), direction (0 forward,
+ * 1 inverse).  Each vector uses the ark-convention root of order 2^log_n.
+ * Vectors must not overlap (a vector may be transformed in place). */
+typedef struct {
+  uint64_t in_offset;
+  uint64_t out_offset;
+  uint32_t log_n;
+  uint32_t inverse;
+} omr_batch_item;
+
+typedef struct omr_batch_plan omr_batch_plan;
+
+/* Plans a batch: pass schedules per vector, tile task lists per pass,
+ * twiddle tables and intermediate buffers. */
+omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_batch_item *items,
+                                 uint32_t count, omr_batch_plan **out);
+omr_status omr_batch_plan_destroy(omr_batch_plan *plan);
+/* Total elements, kernel launches per execution and tile tasks of a plan. */
+omr_status omr_batch_plan_info(const omr_batch_plan *plan, uint64_t *total_elems,
+                               uint32_t *nlaunch, uint32_t *ntasks);
+/* Runs the batch: one persistent-kernel launch per pass index. */
+omr_status omr_batch_execute(omr_batch_plan *plan, const void *d_in, void *d_out, void *stream);
+/* Same work, captured once into a hipGraph (per d_in/d_out pair) and replayed. */
+omr_status omr_batch_execute_graph(omr_batch_plan *plan, const void *d_in, void *d_out,
+                                   void *stream);
+/* Baseline form: the single-vector transform launched once per vector. */
+omr_status omr_batch_execute_plain(omr_batch_plan *plan, const void *d_in, void *d_out,
+                                   void *stream);
+/* Largest pass radix (2^4 .. 2^12) the batch planner gives multi-pass vectors
+ * of later plans (default 2^9). */
+omr_status omr_batch_set_max_log_radix(int log_radix);
+
 /* Elementwise device field arithmetic on canonical arrays (test hook):
  * op 0 add, 1 sub, 2 mul, 3 multiplier-form round trip of a, 4 a * b via
  * multiplier form. */

@@ -1,0 +1,385 @@
+// batch.hip -- batched transforms of vectors of uneven power-of-two sizes.
+//
+// A plan splits every vector into Stockham passes of radix 2^4 .. 2^12 and
+// lists the 4096-element tiles of pass i of all vectors as the tasks of
+// launch i of the persistent kernel k_batch_tiles (batch_kernel.h).  Pass
+// boundaries are kernel boundaries: a pass reads what other workgroups (on
+// other XCDs, with their own L2) wrote in the previous pass, which coarse-
+// grained device memory makes visible only across launches.  The launches
+// are captured once into a hipGraph and replayed.  The plain form (one
+// launch sequence per vector) is kept for comparison.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "batch_kernel.h"
+#include "dispatch.h"
+#include "internal.h"
+
+using namespace omr;
+
+struct omr_batch_plan {
+  omr_ctx *ctx = nullptr;
+  int field = 0;
+  std::vector<omr_batch_item> items;
+  std::vector<std::vector<int>> sched;  // per item pass radices
+  uint32_t nlaunch = 0;
+  std::vector<uint32_t> ntasks, grid;
+  std::vector<void *> d_tasks, d_vecs;
+  void *d_tsets = nullptr;
+  DevBuf scratch[2];
+  uint64_t total_elems = 0;
+  // graph cache
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void *g_in = nullptr;
+  void *g_out = nullptr;
+};
+
+namespace omr {
+
+static hipError_t launch_batch(int f, dim3 grid, hipStream_t s, const BatchTask *t, uint32_t nt,
+                               const BatchVec *v, const BatchTables *tb, BatchBases b) {
+  switch (f) {
+    case kBabyBear:
+      hipLaunchKernelGGL(k_batch_tiles<BabyBear>, grid, dim3(kTileThreads), 0, s, t, nt, v, tb, b);
+      break;
+    case kKoalaBear:
+      hipLaunchKernelGGL(k_batch_tiles<KoalaBear>, grid, dim3(kTileThreads), 0, s, t, nt, v, tb, b);
+      break;
+    case kGoldilocks:
+      hipLaunchKernelGGL(k_batch_tiles<Goldilocks>, grid, dim3(kTileThreads), 0, s, t, nt, v, tb, b);
+      break;
+    case kTeddyBear:
+      hipLaunchKernelGGL(k_batch_tiles<TeddyBear>, grid, dim3(kTileThreads), 0, s, t, nt, v, tb, b);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Largest pass radix the batch planner uses for multi-pass vectors: 2^9
+// keeps >= 8 columns (64 B runs for Goldilocks) per 4096-element tile.
+static int g_batch_max_log_radix = 9;
+
+// Pass radices of one vector of size 2^logn inside a batch.
+static void batch_schedule(uint32_t logn, std::vector<int> &out) {
+  out.clear();
+  if (logn <= 12) {
+    out.push_back((int)logn);
+    return;
+  }
+  const int cap = g_batch_max_log_radix;
+  const int parts = ((int)logn + cap - 1) / cap;
+  const int base = (int)logn / parts, rem = (int)logn % parts;
+  for (int i = 0; i < parts; i++) out.push_back(base + (i < rem ? 1 : 0));
+}
+
+}  // namespace omr
+
+extern "C" omr_status omr_batch_set_max_log_radix(int log_radix) {
+  if (log_radix < 4 || log_radix > 12) return OMR_E_SCHEDULE;
+  g_batch_max_log_radix = log_radix;
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_batch_plan_destroy(omr_batch_plan *p) {
+  if (!p) return OMR_E_NULL;
+  (void)hipDeviceSynchronize();
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  for (void *q : p->d_tasks) (void)hipFree(q);
+  for (void *q : p->d_vecs) (void)hipFree(q);
+  if (p->d_tsets) (void)hipFree(p->d_tsets);
+  free_buf(p->scratch[0]);
+  free_buf(p->scratch[1]);
+  delete p;
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_batch_item *items,
+                                            uint32_t count, omr_batch_plan **out) {
+  if (!field_ok(field)) return OMR_E_FIELD;
+  if (!items || !out) return OMR_E_NULL;
+  if (count == 0) return OMR_E_ARG;
+  int two_adicity = 0;
+  omr_field_info(field, nullptr, nullptr, &two_adicity);
+  for (uint32_t i = 0; i < count; i++) {
+    if (items[i].log_n < 4 || (int)items[i].log_n > two_adicity || items[i].log_n > 36)
+      return items[i].log_n > 36 || (int)items[i].log_n > two_adicity ? OMR_E_TWO_ADICITY : OMR_E_SIZE;
+    if (items[i].inverse > 1) return OMR_E_ARG;
+  }
+  if (!ctx) return OMR_E_NULL;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  omr_batch_plan *p = new (std::nothrow) omr_batch_plan();
+  if (!p) return OMR_E_ALLOC;
+  p->ctx = ctx;
+  p->field = field;
+  p->items.assign(items, items + count);
+  p->sched.resize(count);
+  const int es = field_bytes(field);
+
+  // twiddle-table sets, one per (log n, direction)
+  std::map<std::pair<uint32_t, uint32_t>, uint16_t> tset_of;
+  std::vector<BatchTables> tsets;
+  uint64_t scratch_elems = 0;
+  std::vector<uint64_t> scratch_off(count, 0);
+  uint32_t maxpass = 0;
+  omr_status st = OMR_OK;
+  for (uint32_t i = 0; i < count; i++) {
+    const uint32_t logn = items[i].log_n;
+    batch_schedule(logn, p->sched[i]);
+    maxpass = std::max<uint32_t>(maxpass, (uint32_t)p->sched[i].size());
+    p->total_elems += 1ull << logn;
+    if (p->sched[i].size() >= 2) {
+      scratch_off[i] = scratch_elems;
+      scratch_elems += 1ull << logn;
+    }
+    auto key = std::make_pair(logn, items[i].inverse);
+    if (!tset_of.count(key)) {
+      uint64_t w;
+      omr_root_of_unity(field, 1ull << logn, &w);
+      BatchTables bt{};
+      if (items[i].inverse) {
+        w = finv_host(field, w);
+        bt.scale_w = to_w_host(field, finv_host(field, (1ull << logn) % field_modulus(field)));
+      }
+      const TwTables *tab;
+      if ((st = get_tables(ctx, field, logn, w, &tab)) != OMR_OK) break;
+      bt.lo = tab->lo;
+      bt.hi = tab->hi;
+      bt.root = tab->root;
+      bt.lo_bits = tab->lo_bits;
+      bt.log_rmax = tab->log_rmax;
+      tset_of[key] = (uint16_t)tsets.size();
+      tsets.push_back(bt);
+    }
+  }
+  if (st == OMR_OK && tsets.size() > 65535) st = OMR_E_ARG;
+  if (st == OMR_OK && scratch_elems) {
+    if ((st = ensure_buf(p->scratch[0], scratch_elems * es)) == OMR_OK)
+      st = ensure_buf(p->scratch[1], scratch_elems * es);
+  }
+  if (st == OMR_OK && hipMalloc(&p->d_tsets, tsets.size() * sizeof(BatchTables)) != hipSuccess) st = OMR_E_ALLOC;
+  if (st == OMR_OK && hipMemcpy(p->d_tsets, tsets.data(), tsets.size() * sizeof(BatchTables),
+                                hipMemcpyHostToDevice) != hipSuccess)
+    st = OMR_E_DEVICE;
+
+  // vectors ordered by size, largest first (the tasks inherit this order)
+  std::vector<uint32_t> order(count);
+  for (uint32_t i = 0; i < count; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return items[a].log_n > items[b].log_n; });
+
+  int cus = 256;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) cus = prop.multiProcessorCount;
+  }
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_tiles<Goldilocks>, kTileThreads, 0) !=
+          hipSuccess || occ < 1)
+    occ = 2;
+
+  for (uint32_t pass = 0; st == OMR_OK && pass < maxpass; pass++) {
+    std::vector<BatchTask> tasks;
+    std::vector<BatchVec> vecs;
+    // vectors taking part in this pass, grouped by (log n, direction) so that
+    // small whole vectors can share a tile
+    uint32_t i0 = 0;
+    while (i0 < count) {
+      uint32_t i1 = i0;
+      const uint32_t ln = items[order[i0]].log_n;
+      while (i1 < count && items[order[i1]].log_n == ln) i1++;
+      for (uint32_t dir = 0; dir < 2; dir++) {
+        std::vector<uint32_t> grp;
+        for (uint32_t k = i0; k < i1; k++)
+          if (items[order[k]].inverse == dir && p->sched[order[k]].size() > pass) grp.push_back(order[k]);
+        if (grp.empty()) continue;
+        const std::vector<int> &sc = p->sched[grp[0]];
+        const int np = (int)sc.size();
+        int consumed = 0;
+        for (uint32_t q = 0; q <= pass; q++) consumed += sc[q];
+        const int logR = sc[pass];
+        const int logS = (int)ln - consumed;
+        const int logC = 12 - logR;
+        const int logCols = (int)ln - logR;
+        const uint16_t ts = tset_of[std::make_pair(ln, dir)];
+        uint8_t flags = 0;
+        if (pass > 0) flags |= kTwiddle;
+        if (pass == (uint32_t)np - 1 && dir) flags |= kScale;
+        const uint32_t vbase = (uint32_t)vecs.size();
+        for (uint32_t v : grp) {
+          BatchVec bv{};
+          if (pass == 0) {
+            bv.src_buf = 0;
+            bv.src_off = items[v].in_offset;
+          } else {
+            bv.src_buf = (pass % 2 == 1) ? 2 : 3;
+            bv.src_off = scratch_off[v];
+          }
+          if (pass == (uint32_t)np - 1) {
+            bv.dst_buf = 1;
+            bv.dst_off = items[v].out_offset;
+          } else {
+            bv.dst_buf = (pass % 2 == 0) ? 2 : 3;
+            bv.dst_off = scratch_off[v];
+          }
+          vecs.push_back(bv);
+        }
+        if (logCols >= logC) {  // tiles within one vector
+          const uint32_t ntile = 1u << (logCols - logC);
+          for (uint32_t gi = 0; gi < grp.size(); gi++)
+            for (uint32_t tl = 0; tl < ntile; tl++) {
+              BatchTask t{};
+              t.vec0 = vbase + gi;
+              t.tile = tl;
+              t.logN = (uint8_t)ln;
+              t.logR = (uint8_t)logR;
+              t.logS = (uint8_t)logS;
+              t.flags = flags;
+              t.tset = ts;
+              t.nvec = 1;
+              tasks.push_back(t);
+            }
+        } else {  // several whole vectors per tile
+          const uint32_t per = 1u << (logC - logCols);
+          for (uint32_t gi = 0; gi < grp.size(); gi += per) {
+            BatchTask t{};
+            t.vec0 = vbase + gi;
+            t.tile = 0;
+            t.logN = (uint8_t)ln;
+            t.logR = (uint8_t)logR;
+            t.logS = (uint8_t)logS;
+            t.flags = flags;
+            t.tset = ts;
+            t.nvec = (uint16_t)std::min<uint32_t>(per, (uint32_t)grp.size() - gi);
+            tasks.push_back(t);
+          }
+        }
+      }
+      i0 = i1;
+    }
+    void *dt = nullptr, *dv = nullptr;
+    if (hipMalloc(&dt, std::max<size_t>(1, tasks.size()) * sizeof(BatchTask)) != hipSuccess ||
+        hipMalloc(&dv, std::max<size_t>(1, vecs.size()) * sizeof(BatchVec)) != hipSuccess) {
+      st = OMR_E_ALLOC;
+      if (dt) (void)hipFree(dt);
+      break;
+    }
+    if (hipMemcpy(dt, tasks.data(), tasks.size() * sizeof(BatchTask), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dv, vecs.data(), vecs.size() * sizeof(BatchVec), hipMemcpyHostToDevice) != hipSuccess)
+      st = OMR_E_DEVICE;
+    p->d_tasks.push_back(dt);
+    p->d_vecs.push_back(dv);
+    p->ntasks.push_back((uint32_t)tasks.size());
+    p->grid.push_back((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tasks.size(), (uint64_t)cus * occ)));
+  }
+  p->nlaunch = (uint32_t)p->ntasks.size();
+  if (st != OMR_OK) {
+    omr_batch_plan_destroy(p);
+    return st;
+  }
+  *out = p;
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_batch_plan_info(const omr_batch_plan *p, uint64_t *total_elems,
+                                          uint32_t *nlaunch, uint32_t *ntasks) {
+  if (!p) return OMR_E_NULL;
+  if (total_elems) *total_elems = p->total_elems;
+  if (nlaunch) *nlaunch = p->nlaunch;
+  if (ntasks) {
+    uint32_t s = 0;
+    for (uint32_t n : p->ntasks) s += n;
+    *ntasks = s;
+  }
+  return OMR_OK;
+}
+
+static omr_status batch_launches(omr_batch_plan *p, const void *d_in, void *d_out, hipStream_t s) {
+  BatchBases b;
+  b.buf[0] = d_in;
+  b.buf[1] = d_out;
+  b.buf[2] = p->scratch[0].p;
+  b.buf[3] = p->scratch[1].p;
+  for (uint32_t i = 0; i < p->nlaunch; i++) {
+    if (p->ntasks[i] == 0) continue;
+    if (launch_batch(p->field, dim3(p->grid[i]), s, (const BatchTask *)p->d_tasks[i], p->ntasks[i],
+                     (const BatchVec *)p->d_vecs[i], (const BatchTables *)p->d_tsets, b) != hipSuccess)
+      return OMR_E_DEVICE;
+  }
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_batch_execute(omr_batch_plan *p, const void *d_in, void *d_out, void *stream) {
+  if (!p || !d_in || !d_out) return OMR_E_NULL;
+  if (hipSetDevice(p->ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  return batch_launches(p, d_in, d_out, (hipStream_t)stream);
+}
+
+extern "C" omr_status omr_batch_execute_graph(omr_batch_plan *p, const void *d_in, void *d_out,
+                                              void *stream) {
+  if (!p || !d_in || !d_out) return OMR_E_NULL;
+  if (hipSetDevice(p->ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  if (!p->exec || p->g_in != d_in || p->g_out != d_out) {
+    if (p->exec) {
+      (void)hipGraphExecDestroy(p->exec);
+      p->exec = nullptr;
+    }
+    if (p->graph) {
+      (void)hipGraphDestroy(p->graph);
+      p->graph = nullptr;
+    }
+    if (!p->cap_stream && hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking) != hipSuccess)
+      return OMR_E_DEVICE;
+    if (hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return OMR_E_DEVICE;
+    omr_status st = batch_launches(p, d_in, d_out, p->cap_stream);
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(p->cap_stream, &g) != hipSuccess || st != OMR_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st != OMR_OK ? st : OMR_E_DEVICE;
+    }
+    p->graph = g;
+    if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) return OMR_E_DEVICE;
+    p->g_in = d_in;
+    p->g_out = d_out;
+  }
+  return hipGraphLaunch(p->exec, (hipStream_t)stream) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+}
+
+// Baseline form: one launch sequence per vector (the engine's plain
+// single-vector transform, one call per vector).
+extern "C" omr_status omr_batch_execute_plain(omr_batch_plan *p, const void *d_in, void *d_out,
+                                              void *stream) {
+  if (!p || !d_in || !d_out) return OMR_E_NULL;
+  omr_ctx *ctx = p->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const int es = field_bytes(p->field);
+  for (const omr_batch_item &it : p->items) {
+    uint64_t w;
+    omr_root_of_unity(p->field, 1ull << it.log_n, &w);
+    Schedule s;
+    if (!tuned_schedule(p->field, it.log_n, s)) default_schedule(p->field, it.log_n, s);
+    XformOpts o;
+    if (it.inverse) {
+      w = finv_host(p->field, w);
+      o.scale = true;
+      o.scale_w = to_w_host(p->field, finv_host(p->field, (1ull << it.log_n) % field_modulus(p->field)));
+    }
+    const uint64_t n = 1ull << it.log_n;
+    omr_status st = run_transform(ctx, p->field, (const char *)d_in + it.in_offset * es,
+                                  (char *)d_out + it.out_offset * es, it.log_n, w, s, 1, n, n, o,
+                                  (hipStream_t)stream);
+    if (st != OMR_OK) return st;
+  }
+  return OMR_OK;
+}

@@ -1,0 +1,169 @@
+// batch_kernel.h -- persistent Stockham tile kernel with a runtime radix,
+// for batches of vectors of uneven power-of-two sizes.
+//
+// Every task is one 4096-element tile of one Stockham pass (the pass
+// semantics of engine.h) of one vector, or of several whole small vectors
+// when a vector has fewer than C columns.  All tasks use 256 threads and a
+// register radix of 16; the pass radix 2^logR (4 <= logR <= 12), the vector
+// size and the twiddle tables are per-task data, so one kernel serves every
+// size in the batch.  Workgroups walk the task list with a grid stride
+// (tasks are sorted by cost, largest first).
+#pragma once
+#include "engine.h"
+
+namespace omr {
+
+constexpr int kTileElems = 4096;
+constexpr int kTileThreads = 256;  // kTileElems / 16
+
+struct BatchTask {
+  uint32_t vec0;   // first vector of the task (index into the pass's vector table)
+  uint32_t tile;   // tile index within the vector (0 when the task holds several vectors)
+  uint8_t logN, logR, logS, flags;  // flags: kTwiddle | kScale
+  uint16_t tset;   // twiddle-table set
+  uint16_t nvec;   // vectors held by the task (columns of vectors >= nvec are idle)
+};
+
+struct BatchVec {  // one vector in one pass: buffer ids (0 in, 1 out, 2 A, 3 B) and offsets
+  uint64_t src_off, dst_off;
+  uint32_t src_buf, dst_buf;
+};
+
+struct BatchTables {  // tables of one root of order 2^logN, multiplier form
+  const void *lo, *hi, *root;
+  uint64_t scale_w;
+  uint32_t lo_bits, log_rmax;
+};
+
+struct BatchBases {
+  const void *buf[4];
+};
+
+template <class F, int LEI>
+__device__ __forceinline__ void gen_stage(typename F::T (&x)[16], const typename F::T (&wE)[9],
+                                          const typename F::T *lds, const typename F::T *root,
+                                          uint32_t t, uint32_t c, uint32_t logC, uint32_t RE,
+                                          uint32_t logs, uint32_t root_shift) {
+  using T = typename F::T;
+  constexpr int EI = 1 << LEI;
+#pragma unroll
+  for (int g = 0; g < 16 / EI; g++) {
+    const uint32_t tt = t + g * RE;
+    const uint32_t ps = (tt >> logs) << logs;
+    const uint32_t jj = tt & ((1u << logs) - 1);
+#pragma unroll
+    for (int kk = 0; kk < EI; kk++) {
+      const uint32_t row = ps * EI + ((uint32_t)kk << logs) + jj;
+      T v = lds[lpad((row << logC) + c)];
+      const uint32_t e = (uint32_t)kk * ps;
+      if (kk != 0 && e != 0) v = F::mulw(v, root[e << root_shift]);
+      x[g * EI + kk] = v;
+    }
+  }
+  GroupDft<F, 4, LEI, 16 / EI>::run(x, wE);
+}
+
+template <class F>
+__global__ void __launch_bounds__(kTileThreads)
+k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchVec *__restrict__ vecs,
+              const BatchTables *__restrict__ tsets, BatchBases bases) {
+  using T = typename F::T;
+  __shared__ T lds[kTileElems + kTileElems / 32 + 1];
+  const uint32_t tid = threadIdx.x;
+
+  for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+    const BatchTask d = tasks[ti];
+    const BatchTables tb = tsets[d.tset];
+    const T *__restrict__ root = (const T *)tb.root;
+    const uint32_t logN = d.logN, logR = d.logR, logS = d.logS;
+    const uint32_t R = 1u << logR, RE = R >> 4;
+    const uint32_t logC = 12 - logR;  // C = 4096 / R columns per task
+    const uint32_t c = tid & ((1u << logC) - 1), t = tid >> logC;
+    const uint32_t logCols = logN - logR;  // columns per vector
+    const uint32_t logCv = logCols < logC ? logCols : logC;
+    const uint32_t u = c >> logCv;  // vector within the task
+    const bool active = u < d.nvec;
+    const uint64_t q = ((uint64_t)d.tile << logC) + (c & ((1u << logCv) - 1));
+    const uint64_t S = 1ull << logS;
+    const uint64_t p = q >> logS, j = q & (S - 1);
+    const BatchVec bv = vecs[d.vec0 + (active ? u : 0)];
+    const T *__restrict__ src = (const T *)bases.buf[bv.src_buf] + bv.src_off;
+    T *__restrict__ dst = (T *)bases.buf[bv.dst_buf] + bv.dst_off;
+    const uint32_t root_shift = tb.log_rmax - logR;
+
+    T wE[9];
+#pragma unroll
+    for (int k = 0; k <= 8; k++) wE[k] = root[((uint32_t)k * RE) << root_shift];
+
+    T x[16];
+    if (S >= (1ull << logCv)) {
+      const T *base = src + ((p << logR) << logS) + j;
+#pragma unroll
+      for (int m = 0; m < 16; m++) x[m] = active ? base[(uint64_t)(t + m * RE) << logS] : (T)0;
+    } else {
+      // each vector of the task contributes the contiguous block
+      // [q0 R, (q0 + Cv) R) of its data, q0 = tile * C
+      const uint32_t vblk = 1u << (logCv + logR);
+      for (uint32_t i = tid; i < (uint32_t)kTileElems; i += kTileThreads) {
+        const uint32_t uu = i >> (logCv + logR);
+        if (uu >= d.nvec) continue;
+        const uint32_t ii = i & (vblk - 1);
+        const uint32_t jj = ii & (uint32_t)(S - 1);
+        const uint32_t k = (ii >> logS) & (R - 1);
+        const uint32_t pp = ii >> (logS + logR);
+        const BatchVec v2 = vecs[d.vec0 + uu];
+        const T *s2 = (const T *)bases.buf[v2.src_buf] + v2.src_off;
+        const uint32_t col = (uu << logCv) + (pp << logS) + jj;
+        lds[lpad((k << logC) + col)] = s2[((uint64_t)d.tile << (logC + logR)) + ii];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 16; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
+    }
+
+    if (d.flags & kTwiddle) {
+      const T *tlo = (const T *)tb.lo, *thi = (const T *)tb.hi;
+      const uint64_t ps = p << logS;
+      const uint64_t lomask = (1ull << tb.lo_bits) - 1;
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        const uint64_t e = (uint64_t)(t + m * RE) * ps;
+        if (e) x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> tb.lo_bits]));
+      }
+    }
+
+    dft_reg<F, 4, 4, 0>(x, wE);
+    const uint32_t nst = (logR + 3) >> 2;
+    const uint32_t loglast = logR - 4 * (nst - 1);
+    for (uint32_t st = 1; st < nst; st++) {
+      const uint32_t logEi = (st == nst - 1) ? loglast : 4;
+      const uint32_t logs = logR - 4 * st - logEi;
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 16; m++) lds[lpad(((t + m * RE) << logC) + c)] = x[m];
+      __syncthreads();
+      switch (logEi) {
+        case 4: gen_stage<F, 4>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+        case 3: gen_stage<F, 3>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+        case 2: gen_stage<F, 2>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+        default: gen_stage<F, 1>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+      }
+    }
+
+    const uint32_t logNR = logN - logR;
+    const bool scale = (d.flags & kScale) != 0;
+    const T sw = (T)tb.scale_w;
+    const uint32_t ELm = (1u << loglast) - 1;
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const uint32_t g = (uint32_t)m >> loglast, kk = (uint32_t)m & ELm;
+      const uint32_t krow = kk * (R >> loglast) + t + g * RE;
+      T v = x[m];
+      if (scale) v = F::mulw(v, sw);
+      if (active) dst[((uint64_t)krow << logNR) + q] = v;
+    }
+    __syncthreads();  // LDS is reused by the next task
+  }
+}
+
+}  // namespace omr

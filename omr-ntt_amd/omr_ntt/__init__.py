@@ -90,6 +90,13 @@ def lib():
             "omr_default_schedule": (i32, [i32, u64, Pi, Pi]),
             "omr_partial_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, i32, i32, u32, _VP]),
             "omr_field_op_device": (i32, [i32, i32, _VP, _VP, _VP, u64, _VP]),
+            "omr_batch_plan_create": (i32, [_VP, i32, _VP, u32, ctypes.POINTER(_VP)]),
+            "omr_batch_plan_destroy": (i32, [_VP]),
+            "omr_batch_plan_info": (i32, [_VP, P64, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+            "omr_batch_execute": (i32, [_VP, _VP, _VP, _VP]),
+            "omr_batch_execute_graph": (i32, [_VP, _VP, _VP, _VP]),
+            "omr_batch_execute_plain": (i32, [_VP, _VP, _VP, _VP]),
+            "omr_batch_set_max_log_radix": (i32, [i32]),
             "omr_coset_lde_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, _VP]),
             "omr_coset_decode_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, _VP, _VP]),
         }
@@ -247,6 +254,49 @@ def coset_decode_device(evals, coeffs, field, n, ncols=1, blowup=4, shift=0, non
     _check(lib().omr_coset_decode_device(context(), field.id, _dptr(evals), _dptr(coeffs), n, ncols,
                                          blowup, shift, nz, _stream_ptr()), "omr_coset_decode_device")
     return coeffs
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("in_offset", ctypes.c_uint64), ("out_offset", ctypes.c_uint64),
+                ("log_n", ctypes.c_uint32), ("inverse", ctypes.c_uint32)]
+
+
+class BatchPlan:
+    """Batch of vectors of uneven power-of-two sizes (omr_batch_plan_*).
+    items: list of (in_offset, out_offset, log_n, inverse) in elements."""
+
+    def __init__(self, field, items):
+        self.field = field
+        arr = (BatchItem * len(items))(*[BatchItem(*it) for it in items])
+        self._p = _VP()
+        _check(lib().omr_batch_plan_create(context(), field.id, ctypes.cast(arr, _VP), len(items),
+                                           ctypes.byref(self._p)), "omr_batch_plan_create")
+
+    def info(self):
+        tot, nl, nt = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().omr_batch_plan_info(self._p, ctypes.byref(tot), ctypes.byref(nl), ctypes.byref(nt)))
+        return {"total_elems": tot.value, "launches": nl.value, "tasks": nt.value}
+
+    def execute(self, x, out, mode="graph"):
+        fn = {"graph": lib().omr_batch_execute_graph, "persistent": lib().omr_batch_execute,
+              "plain": lib().omr_batch_execute_plain}[mode]
+        _check(fn(self._p, _dptr(x), _dptr(out), _stream_ptr()), f"omr_batch_execute({mode})")
+        return out
+
+    def close(self):
+        if self._p:
+            lib().omr_batch_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_set_max_log_radix(r):
+    _check(lib().omr_batch_set_max_log_radix(r))
 
 
 def field_op_device(field, op, a, b, out):
