@@ -66,6 +66,51 @@ def cpu_baseline(x, n, budget_s=20.0):
                       f"median of {len(times)} runs"}, out
 
 
+def other_sizes(E, F, torch, np):
+    out = {}
+    for ln in (20, 22):
+        n = 1 << ln
+        nb = max(3, (1 << 26) // n)  # rotate past the Infinity Cache
+        xs = [E.to_device(np.random.default_rng(ln + i).integers(0, F.p, size=n, dtype=np.uint64), F)
+              for i in range(nb)]
+        ys = [E.empty_device(n, F) for _ in range(nb)]
+        fwd = time_device(lambda i: E.ntt_device(xs[i % nb], ys[i % nb], F), 30, 5, torch)
+        inv = time_device(lambda i: E.intt_device(ys[i % nb], xs[i % nb], F), 30, 5, torch)
+        out[f"2^{ln}"] = {"forward_us": fwd * 1e6, "inverse_us": inv * 1e6, "schedule": E.default_schedule(F, n),
+                          "elems_per_s": n / fwd}
+        log(f"bench: 2^{ln} forward {fwd * 1e6:.1f} us inverse {inv * 1e6:.1f} us")
+    return out
+
+
+def batched(E, F, torch, np):
+    """4096 Goldilocks vectors, log2 n ~ U{10..20} (seed 42), mixed forward /
+    inverse: graph-replayed persistent plan vs one launch sequence per vector."""
+    rng = np.random.default_rng(42)
+    logs = rng.integers(10, 21, size=4096)
+    dirs = rng.integers(0, 2, size=4096)
+    items, off = [], 0
+    for ln, d in zip(logs, dirs):
+        items.append((off, off, int(ln), int(d)))
+        off += 1 << int(ln)
+    x = torch.randint(0, 2**62, (off,), dtype=torch.int64, device="cuda")
+    y_graph = torch.empty_like(x)
+    y_plain = torch.empty_like(x)
+    plan = E.BatchPlan(F, items)
+    info = plan.info()
+    t_graph = time_device(lambda i: plan.execute(x, y_graph, "graph"), 5, 2, torch)
+    t_pers = time_device(lambda i: plan.execute(x, y_graph, "persistent"), 5, 1, torch)
+    t_plain = time_device(lambda i: plan.execute(x, y_plain, "plain"), 3, 1, torch)
+    same = bool(torch.equal(y_graph, y_plain))
+    plan.close()
+    res = {"vectors": 4096, "total_elems": info["total_elems"], "launches": info["launches"],
+           "tasks": info["tasks"], "graph_ms": t_graph * 1e3, "persistent_ms": t_pers * 1e3,
+           "plain_ms": t_plain * 1e3, "speedup_vs_plain": t_plain / t_graph,
+           "elems_per_s": info["total_elems"] / t_graph, "graph_equals_plain": same}
+    log(f"bench: batched graph {t_graph * 1e3:.2f} ms plain {t_plain * 1e3:.2f} ms "
+        f"speedup {t_plain / t_graph:.2f}x equal={same}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +118,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--logn", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 2^20/2^22 sizes and the batched uneven-size workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path "
@@ -142,12 +189,19 @@ def main():
         "inverse_us": sec_inv * 1e6,
         "bit_exact": {"gpu_round_trip": round_trip},
     }
+    if not args.no_extras:
+        del xs, ys, back
+        torch.cuda.empty_cache()
+        result["sizes_us"] = other_sizes(E, F, torch, np)
+        result["batched"] = batched(E, F, torch, np)
     if not args.no_cpu:
         cb, cpu_out = cpu_baseline(host_x, n)
         result["cpu_baseline"] = cb
-        E.ntt_device(xs[0], ys[0], F)
+        x0 = E.to_device(host_x, F)
+        y0 = E.empty_device(n, F)
+        E.ntt_device(x0, y0, F)
         result["bit_exact"]["gpu_vs_cpu_restatement"] = bool(
-            np.array_equal(E.from_device(ys[0], F), cpu_out))
+            np.array_equal(E.from_device(y0, F), cpu_out))
         log(f"bench: cpu baseline {cb['value']:.3e} elems/s on {cb['cores']} threads")
     print(json.dumps(result), flush=True)
 

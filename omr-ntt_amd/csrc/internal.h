@@ -77,6 +77,15 @@ void default_schedule(int f, uint32_t logN, Schedule &s);
 omr_status variant_schedule(int f, int variant, uint32_t logN, Schedule &s);
 // Autotuned schedule for (field, logN) if the tuning table has one.
 bool tuned_schedule(int f, uint32_t logN, Schedule &s);
+void set_tuned_schedule(int f, uint32_t logN, const Schedule &s);
+// Autotuned partial-output method for (field, logN, log out_len), if any.
+bool tuned_partial(int f, uint32_t logN, uint32_t logOut, int &method, uint32_t &log_n1);
+
+// Schedule-class vocabulary shared by the autotuner and the cost model.
+constexpr int kFullClasses = 30;
+constexpr int kNumClasses = 36;
+bool class_schedule(int f, int cls, uint32_t logN, Schedule &s);
+bool class_partial(int f, int cls, uint32_t logN, uint32_t logOut, int &method, uint32_t &log_n1);
 
 omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
                          uint64_t root, const Schedule &sch, uint64_t batch, uint64_t in_stride,

@@ -86,7 +86,10 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
                          m, m, o, stream);
   }
 
-  if (method == 0) {
+  if (method == 0 && (out_len & (out_len - 1)) == 0 &&
+      tuned_partial(f, logN, ilog2u(out_len), method, log_n1)) {
+    // autotuned choice
+  } else if (method == 0) {
     if (out_len <= 4) method = 1;
     else if (out_len <= (1ull << 13)) method = 2;
     else method = 3;

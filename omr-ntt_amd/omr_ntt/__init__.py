@@ -97,6 +97,29 @@ def lib():
             "omr_batch_execute_graph": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_execute_plain": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_set_max_log_radix": (i32, [i32]),
+            "omr_num_schedule_classes": (i32, []),
+            "omr_schedule_class": (i32, [i32, u64, u64, i32, Pi, Pi, Pi, Pi, ctypes.POINTER(u32)]),
+            "omr_default_class": (i32, [i32, u64, u64]),
+            "omr_tune_classes": (i32, [_VP, i32, u64, u64, u64, i32, ctypes.POINTER(ctypes.c_float)]),
+            "omr_tune_set_class": (i32, [i32, u64, u64, i32]),
+            "omr_tune_clear": (i32, []),
+            "omr_tune_save": (i32, [ctypes.c_char_p]),
+            "omr_tune_load": (i32, [ctypes.c_char_p]),
+            "omr_cost_model_create": (i32, [_VP, u32, ctypes.POINTER(_VP)]),
+            "omr_cost_model_destroy": (i32, [_VP]),
+            "omr_cost_model_num_params": (i32, []),
+            "omr_cost_model_num_inputs": (i32, []),
+            "omr_cost_model_num_outputs": (i32, []),
+            "omr_cost_model_get_params": (i32, [_VP, _VP]),
+            "omr_cost_model_set_params": (i32, [_VP, _VP]),
+            "omr_cost_model_features": (i32, [i32, u64, u64, u64, _VP]),
+            "omr_cost_model_forward": (i32, [_VP, _VP, u32, _VP]),
+            "omr_cost_model_gradients": (i32, [_VP, _VP, _VP, _VP, u32, _VP, _VP]),
+            "omr_cost_model_train": (i32, [_VP, _VP, _VP, _VP, u32, u32, ctypes.c_float, _VP]),
+            "omr_cost_model_choose": (i32, [_VP, i32, u64, u64, u64, Pi]),
+            "omr_cost_model_save": (i32, [_VP, ctypes.c_char_p]),
+            "omr_cost_model_load": (i32, [_VP, ctypes.c_char_p]),
+            "omr_autotune_shape": (i32, [_VP, _VP, i32, i32, u64, u64, u64, i32, Pi, Pi]),
             "omr_coset_lde_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, _VP]),
             "omr_coset_decode_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, _VP, _VP]),
         }
@@ -297,6 +320,144 @@ class BatchPlan:
 
 def batch_set_max_log_radix(r):
     _check(lib().omr_batch_set_max_log_radix(r))
+
+
+# ---- autotuner and learned cost model ---------------------------------------
+
+
+def num_schedule_classes():
+    return lib().omr_num_schedule_classes()
+
+
+def schedule_class(field, n, out_len, cls):
+    """Concrete schedule of class `cls` for a shape, or None if invalid:
+    {"log_radix": [...], "log_e": e} for full transforms, {"method": m,
+    "log_n1": l} for partial ones."""
+    lr = (ctypes.c_int * 8)()
+    npass, le, meth = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    l1 = ctypes.c_uint32()
+    st = lib().omr_schedule_class(field.id, n, out_len, cls, lr, ctypes.byref(npass), ctypes.byref(le),
+                                  ctypes.byref(meth), ctypes.byref(l1))
+    if st == 11:
+        return None
+    _check(st, "omr_schedule_class")
+    if out_len == n:
+        return {"log_radix": [lr[i] for i in range(npass.value)], "log_e": le.value}
+    return {"method": meth.value, "log_n1": l1.value}
+
+
+def default_class(field, n, out_len):
+    return lib().omr_default_class(field.id, n, out_len)
+
+
+def tune_classes(field, n, out_len=None, batch=1, reps=5):
+    """Median microseconds of every schedule class (-1 where invalid)."""
+    out_len = n if out_len is None else out_len
+    t = (ctypes.c_float * num_schedule_classes())()
+    _check(lib().omr_tune_classes(context(), field.id, n, out_len, batch, reps, t), "omr_tune_classes")
+    return np.array(t[:], dtype=np.float64)
+
+
+def tune_set_class(field, n, out_len, cls):
+    _check(lib().omr_tune_set_class(field.id, n, out_len, cls), "omr_tune_set_class")
+
+
+def tune_clear():
+    _check(lib().omr_tune_clear())
+
+
+def tune_save(path):
+    _check(lib().omr_tune_save(path.encode()), "omr_tune_save")
+
+
+def tune_load(path):
+    _check(lib().omr_tune_load(path.encode()), "omr_tune_load")
+
+
+def features(field, n, out_len, batch):
+    x = np.zeros(4, dtype=np.float32)
+    _check(lib().omr_cost_model_features(field.id, n, out_len, batch, x.ctypes.data_as(_VP)))
+    return x
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+class CostModel:
+    """The engine's 4 -> 64 -> 64 -> 36 MLP (HIP forward/backward kernels)."""
+
+    def __init__(self, seed=0):
+        self._p = _VP()
+        _check(lib().omr_cost_model_create(context(), seed, ctypes.byref(self._p)), "omr_cost_model_create")
+        self.n_params = lib().omr_cost_model_num_params()
+        self.n_out = lib().omr_cost_model_num_outputs()
+
+    def params(self):
+        out = np.empty(self.n_params, dtype=np.float32)
+        _check(lib().omr_cost_model_get_params(self._p, out.ctypes.data_as(_VP)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.n_params
+        _check(lib().omr_cost_model_set_params(self._p, p.ctypes.data_as(_VP)))
+
+    def forward(self, X):
+        X = _f32(X).reshape(-1, 4)
+        Y = np.empty((X.shape[0], self.n_out), dtype=np.float32)
+        _check(lib().omr_cost_model_forward(self._p, X.ctypes.data_as(_VP), X.shape[0], Y.ctypes.data_as(_VP)))
+        return Y
+
+    def gradients(self, X, T, M):
+        X, T, M = _f32(X).reshape(-1, 4), _f32(T), _f32(M)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(lib().omr_cost_model_gradients(self._p, X.ctypes.data_as(_VP), T.ctypes.data_as(_VP),
+                                              M.ctypes.data_as(_VP), X.shape[0], g.ctypes.data_as(_VP),
+                                              ctypes.byref(loss)), "omr_cost_model_gradients")
+        return g, loss.value
+
+    def train(self, X, T, M, steps=2000, lr=3e-3):
+        X, T, M = _f32(X).reshape(-1, 4), _f32(T), _f32(M)
+        loss = ctypes.c_float()
+        _check(lib().omr_cost_model_train(self._p, X.ctypes.data_as(_VP), T.ctypes.data_as(_VP),
+                                          M.ctypes.data_as(_VP), X.shape[0], steps, lr, ctypes.byref(loss)),
+               "omr_cost_model_train")
+        return loss.value
+
+    def choose(self, field, n, out_len=None, batch=1):
+        c = ctypes.c_int()
+        _check(lib().omr_cost_model_choose(self._p, field.id, n, n if out_len is None else out_len, batch,
+                                           ctypes.byref(c)), "omr_cost_model_choose")
+        return c.value
+
+    def save(self, path):
+        _check(lib().omr_cost_model_save(self._p, path.encode()), "omr_cost_model_save")
+
+    def load(self, path):
+        _check(lib().omr_cost_model_load(self._p, path.encode()), "omr_cost_model_load")
+
+    def close(self):
+        if self._p:
+            lib().omr_cost_model_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_shape(field, n, out_len=None, batch=1, model=None, use_model=True, reps=5):
+    """Installs a schedule for the shape: the model's choice (no timing) or
+    exhaustive timing of every class.  Returns (class, classes_timed)."""
+    c, timed = ctypes.c_int(), ctypes.c_int()
+    _check(lib().omr_autotune_shape(context(), model._p if model else None, 1 if use_model else 0, field.id, n,
+                                    n if out_len is None else out_len, batch, reps, ctypes.byref(c),
+                                    ctypes.byref(timed)), "omr_autotune_shape")
+    return c.value, timed.value
 
 
 def field_op_device(field, op, a, b, out):

@@ -1,0 +1,53 @@
+"""CPU checks of the schedule-class vocabulary, the cost-model features and
+the float64 MLP oracle (pinned by central finite differences)."""
+import numpy as np
+import pytest
+
+import mlp_oracle as MO
+import omr_ntt as E
+
+
+def test_mlp_oracle_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    K = 36
+    P = rng.normal(0, 0.3, size=64 * 4 + 64 + 64 * 64 + 64 + K * 64 + K)
+    X = rng.uniform(0, 1, size=(7, 4))
+    T = rng.normal(0, 1, size=(7, K))
+    M = (rng.uniform(0, 1, size=(7, K)) > 0.3).astype(np.float64)
+    g, _ = MO.gradients(P, X, T, M, K)
+    h = 1e-6
+    for idx in rng.choice(P.size, size=60, replace=False):
+        e = np.zeros_like(P)
+        e[idx] = h
+        fd = (MO.loss(P + e, X, T, M, K) - MO.loss(P - e, X, T, M, K)) / (2 * h)
+        assert abs(fd - g[idx]) <= 1e-6 + 1e-5 * abs(fd), idx
+
+
+def test_schedule_classes_are_valid_decompositions():
+    assert E.num_schedule_classes() == 36
+    for F in E.FIELDS:
+        maxlr = 12 if F.bytes == 8 else 13
+        for ln in range(4, min(F.TWO_ADICITY, 30) + 1):
+            n = 1 << ln
+            valid = 0
+            for c in range(30):
+                s = E.schedule_class(F, n, n, c)
+                if s is None:
+                    continue
+                valid += 1
+                assert sum(s["log_radix"]) == ln
+                assert all(1 <= r <= maxlr for r in s["log_radix"])
+                assert 2 <= s["log_e"] <= 4
+            assert valid >= 1, (F, ln)
+            d = E.default_class(F, n, n)
+            assert d >= 0, (F, ln)
+            assert E.schedule_class(F, n, n, d)["log_radix"] == E.default_schedule(F, n)
+            for lo in range(0, ln):
+                pc = [c for c in range(30, 36) if E.schedule_class(F, n, 1 << lo, c) is not None]
+                assert 35 in pc  # the pruned full transform is always available
+                assert E.default_class(F, n, 1 << lo) in pc
+
+
+def test_features():
+    x = E.features(E.Goldilocks, 1 << 24, 1 << 8, 16)
+    assert np.allclose(x, [2 / 3, 24 / 32, 8 / 32, 4 / 16])

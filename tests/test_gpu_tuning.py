@@ -1,0 +1,117 @@
+"""GPU tests of the autotuner and the learned cost model: every schedule a
+tuner or the model can pick is bit-exact; HIP gradients match the float64
+oracle; the trained model's held-out regret against the exhaustive autotuner
+and its gain over the untuned default."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+import mlp_oracle as MO  # noqa: E402
+import omr_ntt as E  # noqa: E402
+import oracle as O  # noqa: E402
+from omr_ntt import costmodel as CM  # noqa: E402
+
+from _gpu import inputs  # noqa: E402
+
+
+@pytest.mark.parametrize("F,ln", [(E.Goldilocks, 16), (E.Goldilocks, 20), (E.BabyBear, 18), (E.TeddyBear, 14)])
+def test_every_full_class_is_bit_exact(F, ln):
+    n = 1 << ln
+    x = inputs(F, n, ln)
+    exp = O.ntt(F.id, x)
+    t = E.to_device(x, F)
+    out = E.empty_device(n, F)
+    checked = 0
+    for c in range(30):
+        s = E.schedule_class(F, n, n, c)
+        if s is None:
+            continue
+        E.ntt_schedule_device(t, out, F, s["log_radix"], log_e=s["log_e"])
+        assert np.array_equal(E.from_device(out, F), exp), (c, s)
+        checked += 1
+    assert checked >= 3
+
+
+@pytest.mark.parametrize("F", [E.Goldilocks, E.KoalaBear])
+def test_every_partial_class_is_bit_exact(F):
+    n = 1 << 18
+    x = inputs(F, n, 3)
+    X = O.ntt(F.id, x)
+    t = E.to_device(x, F)
+    for lo in (1, 4, 10, 14):
+        m = 1 << lo
+        out = E.empty_device(m, F)
+        for c in range(30, 36):
+            s = E.schedule_class(F, n, m, c)
+            if s is None:
+                continue
+            E.partial_device(t, out, F, m, method=s["method"], log_n1=s["log_n1"])
+            assert np.array_equal(E.from_device(out, F), X[:m]), (lo, c, s)
+
+
+def test_tune_classes_marks_valid_classes():
+    F, n = E.Goldilocks, 1 << 16
+    t = E.tune_classes(F, n, n, 1, reps=2)
+    for c in range(36):
+        valid = E.schedule_class(F, n, n, c) is not None
+        assert (t[c] > 0) == valid, c
+
+
+def test_gpu_gradients_match_oracle():
+    m = E.CostModel(seed=3)
+    rng = np.random.default_rng(1)
+    B, K = 40, m.n_out
+    X = rng.uniform(0, 1, size=(B, 4)).astype(np.float32)
+    T = rng.normal(0, 1, size=(B, K)).astype(np.float32)
+    M = (rng.uniform(0, 1, size=(B, K)) > 0.25).astype(np.float32)
+    P = m.params().astype(np.float64)
+    Y = m.forward(X)
+    Yo, _ = MO.forward(P, X, K)
+    assert np.allclose(Y, Yo, rtol=1e-4, atol=1e-5)
+    g, loss = m.gradients(X, T, M)
+    go, lo = MO.gradients(P, X, T, M, K)
+    # fp32 kernels vs float64 oracle: relative tolerance 1e-4 of the largest gradient
+    assert np.max(np.abs(g - go)) <= 1e-4 * np.max(np.abs(go)) + 1e-7
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    # training reduces the loss and save/load round-trips the parameters
+    before = MO.loss(P, X, T, M, K)
+    m.train(X, T, M, steps=300, lr=3e-3)
+    after = MO.loss(m.params().astype(np.float64), X, T, M, K)
+    assert after < 0.5 * before
+    path = "/tmp/omr_cost_model_test.bin"
+    m.save(path)
+    m2 = E.CostModel(seed=9)
+    m2.load(path)
+    assert np.array_equal(m2.params(), m.params())
+
+
+def test_learned_schedules_regret_and_gain():
+    """Held-out regret of the model vs the exhaustive autotuner, and what it
+    gains over the untuned default (the method without the model)."""
+    shapes = CM.shape_grid(fields=[E.Goldilocks, E.BabyBear], log_ns=range(10, 23), batches=(1, 8),
+                           partial=True, max_elems=1 << 25)
+    res, model = CM.experiment(shapes, reps=3, seed=0, steps=3000)
+    os.makedirs("gpurun_out", exist_ok=True)
+    with open("gpurun_out/costmodel_regret.json", "w") as fh:
+        json.dump(res, fh, indent=1)
+    print("regret model %.4f default %.4f within10 model %.2f default %.2f" % (
+        res["heldout_mean_regret_model"], res["heldout_mean_regret_default"],
+        res["heldout_frac_within_10pct_model"], res["heldout_frac_within_10pct_default"]))
+    assert res["heldout_mean_regret_model"] <= 0.10
+    assert res["heldout_mean_regret_model"] <= res["heldout_mean_regret_default"] + 1e-9
+    # the model picks without timing; the fallback times every class
+    F, n = E.Goldilocks, 1 << 21
+    c_model, timed_model = E.autotune_shape(F, n, model=model, use_model=True)
+    c_ex, timed_ex = E.autotune_shape(F, n, model=model, use_model=False, reps=2)
+    assert timed_model == 0 and timed_ex >= 2
+    # the installed schedule stays bit-exact
+    x = inputs(F, n, 4)
+    t = E.to_device(x, F)
+    E.ntt_device(t, t, F)
+    assert np.array_equal(E.from_device(t, F), O.ntt(F.id, x))
+    E.tune_clear()
