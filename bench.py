@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--logn", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 128 MiB device copy (torch copy_) 10 times, to calibrate "
+                         "rocprof FETCH_SIZE / WRITE_SIZE against a kernel of known traffic")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 2^20/2^22 sizes and the batched uneven-size workload")
     args = ap.parse_args()
@@ -128,6 +131,17 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        a = torch.randint(0, 2**62, (1 << 24,), dtype=torch.int64, device="cuda")
+        bufs = [torch.empty_like(a) for _ in range(3)]
+        for i in range(10):
+            bufs[i % 3].copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": a.numel() * 8, "bytes_written": a.numel() * 8,
+                                               "launches": 10}}), flush=True)
+        return
+
     import omr_ntt as E
 
     F = E.Goldilocks

@@ -177,6 +177,18 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
     if (hipMemcpy(*dp, src, v.size() * es, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
     return OMR_OK;
   };
+  if (f == kGoldilocks) {  // discrete log base 2 of the root of order min(N, 64)
+    t.log_m = std::min<uint32_t>(logN, 6);
+    const uint64_t wm = fpow_host(f, root, 1ull << (logN - t.log_m));
+    uint64_t pw = 1;
+    t.pow2_unit = 1000;
+    for (uint32_t k = 0; k < 192; k++, pw = fmul_host(f, pw, 2))
+      if (pw == wm) {
+        t.pow2_unit = k;
+        break;
+      }
+    if (t.pow2_unit >= 192) return OMR_E_OMEGA;  // unreachable for a root of order <= 64
+  }
   omr_status st;
   if ((st = upload(lo, &t.lo)) != OMR_OK) return st;
   if ((st = upload(hi, &t.hi)) != OMR_OK) return st;
@@ -346,6 +358,8 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.root = tab->root;
     a.lo_bits = tab->lo_bits;
     a.root_shift = tab->log_rmax - logR;
+    a.pow2_unit = tab->pow2_unit;
+    a.log_m = tab->log_m;
     a.logN = logN;
     a.logS = logN - consumed;
     a.logC = logC;

@@ -37,6 +37,7 @@ struct PassArgs {
   uint32_t logN, logS, logC, lo_bits, root_shift;
   uint32_t flags;     // kTwiddle | kScale | kZeroTail
   uint32_t logValid;  // with kZeroTail: input positions >= 2^logValid read as zero
+  uint32_t pow2_unit, log_m;  // fields with power-of-2 roots: 2^pow2_unit has order 2^log_m
 };
 enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u };
 
@@ -45,6 +46,30 @@ __device__ __forceinline__ constexpr int brev_c(int i) {
   int r = 0;
   for (int b = 0; b < LOGX; b++) r |= ((i >> b) & 1) << (LOGX - 1 - b);
   return r;
+}
+
+// Multiply by a register-DFT constant: wE holds multiplier-form roots, or
+// (fields whose small roots are powers of 2) the shift amounts of 2^k.
+template <class F>
+__device__ __forceinline__ typename F::T const_mul(typename F::T d, typename F::T c) {
+  if constexpr (F::kPow2Roots) return F::mul_pow2(d, (uint32_t)c);
+  else return F::mulw(d, c);
+}
+
+// Register-DFT constants w_E^k, k <= E/2: multiplier-form values loaded from
+// the root table, or shift amounts (pow2_unit * k * (M/E)) mod 192 where
+// 2^pow2_unit is the root of order M = 2^log_m (M = min(N, 64)).
+template <class F, int LOGE>
+__device__ __forceinline__ void load_consts(typename F::T (&wE)[(1 << LOGE) / 2 + 1], const typename F::T *root,
+                                            uint32_t RE, uint32_t root_shift, uint32_t pow2_unit,
+                                            uint32_t log_m) {
+#pragma unroll
+  for (int k = 0; k <= (1 << LOGE) / 2; k++) {
+    if constexpr (F::kPow2Roots)
+      wE[k] = (typename F::T)((pow2_unit * ((uint32_t)k << (log_m - LOGE))) % 192u);
+    else
+      wE[k] = root[((uint32_t)k * RE) << root_shift];
+  }
 }
 
 // In-register DFT of size X = 2^LOGX on x[OFF .. OFF+X), natural order in
@@ -67,7 +92,7 @@ __device__ __forceinline__ void dft_reg(typename F::T (&x)[1 << LOGE],
         T u = x[OFF + b + j], v = x[OFF + b + j + half];
         x[OFF + b + j] = F::add(u, v);
         T d = F::sub(u, v);
-        x[OFF + b + j + half] = (j == 0) ? d : F::mulw(d, wE[j * (E / len)]);
+        x[OFF + b + j + half] = (j == 0) ? d : const_mul<F>(d, wE[j * (E / len)]);
       }
     }
   }
@@ -173,8 +198,7 @@ k_stockham_pass(PassArgs a) {
   const uint64_t p = q >> logS, j = q & (S - 1);
 
   T wE[E / 2 + 1];
-#pragma unroll
-  for (int k = 0; k <= E / 2; k++) wE[k] = root[((uint32_t)k * RE) << a.root_shift];
+  load_consts<F, LOGE>(wE, root, RE, a.root_shift, a.pow2_unit, a.log_m);
 
   // input position of row k in this column: (p R + k) s + j
   const uint64_t pos0 = ((p << LOGR) << logS) + j;

@@ -74,6 +74,7 @@ struct Mont32 {
   OMR_HD static T from_w(T w) { return mont(w, 1); }
   OMR_HD static T mul(T a, T b) { return mont(mont(a, b), R2); }
   OMR_HD static T one_w() { return R_MOD; }
+  static constexpr bool kPow2Roots = false;
 };
 
 using BabyBear = Mont32<2013265921u, 31, 27, kBabyBear>;
@@ -129,6 +130,24 @@ struct Goldilocks {
   OMR_HD static T to_w(T a) { return a; }
   OMR_HD static T from_w(T w) { return w; }
   OMR_HD static T one_w() { return 1; }
+
+  // 2 has multiplicative order 192 (2^96 = -1), so every root of unity of
+  // order dividing 64 is a power of 2 and multiplying by it is a shift plus
+  // the reduction -- no 64x64 product.
+  static constexpr bool kPow2Roots = true;
+  // x * 2^k for 0 < k < 64
+  OMR_HD static T shl_reduce(T x, uint32_t k) { return reduce128(x << k, x >> (64 - k)); }
+  // x * 2^k mod p for 0 <= k < 192, x canonical
+  OMR_HD static T mul_pow2(T x, uint32_t k) {
+    const bool negate = k >= 96;
+    if (negate) k -= 96;
+    if (k >= 64) {
+      x = shl_reduce(x, 32);
+      k -= 32;
+    }
+    if (k) x = shl_reduce(x, k);
+    return negate ? neg(x) : x;
+  }
 };
 
 // Host helpers on canonical values (uint64_t for every field).

@@ -23,6 +23,8 @@ struct TwTables {
   void *root = nullptr;  // root^(m 2^(logN - log_rmax)), m < 2^log_rmax
   uint32_t lo_bits = 0;
   uint32_t log_rmax = 0;
+  uint32_t pow2_unit = 0;  // Goldilocks: 2^pow2_unit = root^(2^logN / M), M = 2^log_m = min(2^logN, 64)
+  uint32_t log_m = 0;
 };
 
 // A transform schedule: pass radices 2^logR[i] (sum = logN), register radix

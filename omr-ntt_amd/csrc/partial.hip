@@ -40,6 +40,8 @@ static omr_status first_pass(omr_ctx *ctx, int f, const void *in, void *out, uin
   a.root = tab->root;
   a.lo_bits = tab->lo_bits;
   a.root_shift = tab->log_rmax - logR;
+  a.pow2_unit = tab->pow2_unit;
+  a.log_m = tab->log_m;
   a.logN = logN;
   a.logS = logN - logR;
   a.logC = logC;
