@@ -348,8 +348,7 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     int logE = sch.logE;
     if (!has_pass_kernel(logR, logE, es)) logE = auto_log_e(logR);
     if (!has_pass_kernel(logR, logE, es)) return OMR_E_SCHEDULE;
-    const int cmax = pass_cmax(logR, es);
-    const int logC = std::min<int>(ilog2((uint64_t)cmax), (int)logN - logR);
+    const int logC = pass_log_cols((int)logN, logR, logE, es);
     PassArgs a{};
     a.src = X[i];
     a.dst = (void *)X[i + 1];

@@ -15,7 +15,7 @@ inline bool has_pass_kernel(int logR, int logE, int esize) {
   if (logR < 1 || logR > max_log_radix(esize)) return false;
   if (logE < 1 || logE > 4 || logE > logR) return false;
   if (logE == 1) return logR <= 2;
-  return true;
+  return logR - logE <= 10;  // one thread per E elements of a column: <= 1024 threads
 }
 
 // Columns per tile the kernel for (esize, logR) was compiled for.
@@ -24,6 +24,16 @@ inline int pass_cmax(int logR, int esize) {
   if (c < 1) c = 1;
   if (c > 32) c = 32;
   return c;
+}
+
+// Columns per tile actually launched: the compiled maximum, at most the
+// number of columns, and at most 1024 threads per workgroup.
+inline int pass_log_cols(int logN, int logR, int logE, int esize) {
+  int lc = 0;
+  while ((2 << lc) <= pass_cmax(logR, esize)) lc++;
+  if (lc > logN - logR) lc = logN - logR;
+  if (lc > 10 - (logR - logE)) lc = 10 - (logR - logE);
+  return lc < 0 ? 0 : lc;
 }
 
 // Launches one Stockham pass; returns hipSuccess or the launch error.

@@ -130,7 +130,7 @@ struct PassKernel {
   static constexpr int LOGLAST = LOGR - (NSTAGE - 1) * LOGE;  // radix of the last in-tile stage
   static constexpr int CMAX0 = (kLdsBudget / (int)sizeof(T)) >> LOGR;
   static constexpr int CMAX = CMAX0 < 1 ? 1 : (CMAX0 > 32 ? 32 : CMAX0);
-  static constexpr int MAXT = CMAX * RE;
+  static constexpr int MAXT = CMAX * RE > 1024 ? 1024 : CMAX * RE;  // hardware limit
   static constexpr int LDS_N = CMAX * R;
 };
 
@@ -259,6 +259,7 @@ k_stockham_pass(PassArgs a) {
   const uint32_t logNR = a.logN - LOGR;
   const bool scale = (a.flags & kScale) != 0;
   const T sw = (T)a.scale_w;
+  uint32_t nonzero = 0;
 #pragma unroll
   for (int g = 0; g < E / EL; g++) {
 #pragma unroll
@@ -272,9 +273,10 @@ k_stockham_pass(PassArgs a) {
         v = F::mulw(v, F::wmul(plo[oidx & ((1ull << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
       }
       if (oidx < a.store_limit) dst[oidx] = v;
-      else if (a.nonzero_count && v != 0) atomicAdd(a.nonzero_count, 1u);
+      else nonzero += (v != 0);
     }
   }
+  if (a.nonzero_count && nonzero) atomicAdd(a.nonzero_count, nonzero);
 }
 
 }  // namespace omr

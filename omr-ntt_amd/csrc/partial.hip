@@ -31,7 +31,7 @@ static omr_status first_pass(omr_ctx *ctx, int f, const void *in, void *out, uin
   omr_status st = get_tables(ctx, f, logN, omega, &tab);
   if (st != OMR_OK) return st;
   const int logE = logR <= 2 ? logR : std::min(4, logR);
-  const int logC = std::min<int>(ilog2u((uint64_t)pass_cmax(logR, es)), (int)logN - logR);
+  const int logC = pass_log_cols((int)logN, logR, logE, es);
   PassArgs a{};
   a.src = in;
   a.dst = out;
