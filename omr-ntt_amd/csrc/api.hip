@@ -134,6 +134,7 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
       (void)hipFree(kv.second.lo);
       (void)hipFree(kv.second.hi);
       (void)hipFree(kv.second.root);
+      if (kv.second.pow2_sh) (void)hipFree(kv.second.pow2_sh);
     }
     ctx->tables.clear();
   }
@@ -142,7 +143,10 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
   TwTables t{};
   t.lo_bits = std::min<uint32_t>(logN, 12);
   const uint64_t nlo = 1ull << t.lo_bits, nhi = 1ull << (logN - t.lo_bits);
-  t.log_rmax = std::min<uint32_t>(logN, (uint32_t)max_log_radix(es));
+  // subgroup table: root^(m 2^(logN - log_rmax)), m < 2^log_rmax, serves the
+  // in-tile roots of every pass radix and single-lookup inter-pass twiddles
+  // for passes with N/s <= 2^log_rmax
+  t.log_rmax = std::min<uint32_t>(logN, kLogSubgroup);
   const uint64_t nroot = 1ull << t.log_rmax;
   std::vector<uint64_t> lo(nlo), hi(nhi), rt(nroot);
   uint64_t cur = 1;
@@ -189,7 +193,17 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
       }
     if (t.pow2_unit >= 192) return OMR_E_OMEGA;  // unreachable for a root of order <= 64
   }
+  std::vector<uint64_t> sh;
+  if (f == kGoldilocks) {  // shift amounts of the powers of the root of order M = 2^log_m
+    sh.resize(1ull << t.log_m);
+    for (size_t m = 0; m < sh.size(); m++) sh[m] = (t.pow2_unit * m) % 192;
+  }
   omr_status st;
+  if (!sh.empty()) {
+    std::vector<uint32_t> sh32(sh.begin(), sh.end());
+    if (hipMalloc(&t.pow2_sh, sh32.size() * 4) != hipSuccess) return OMR_E_ALLOC;
+    if (hipMemcpy(t.pow2_sh, sh32.data(), sh32.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
+  }
   if ((st = upload(lo, &t.lo)) != OMR_OK) return st;
   if ((st = upload(hi, &t.hi)) != OMR_OK) return st;
   if ((st = upload(rt, &t.root)) != OMR_OK) return st;
@@ -359,6 +373,8 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.root_shift = tab->log_rmax - logR;
     a.pow2_unit = tab->pow2_unit;
     a.log_m = tab->log_m;
+    a.pow2_sh = (const uint32_t *)tab->pow2_sh;
+    a.log_rmax = tab->log_rmax;
     a.logN = logN;
     a.logS = logN - consumed;
     a.logC = logC;
@@ -384,9 +400,10 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
       a.pre_lo = o.pre_lo;
       a.pre_hi = o.pre_hi;
     }
+    a.nvec = (uint32_t)batch;
     dim3 block((1u << logC) << (logR - logE));
-    dim3 grid((unsigned)((N >> logR) >> logC), (unsigned)batch);
-    if (launch_pass(f, logR, logE, a, grid, block, stream) != hipSuccess) return OMR_E_DEVICE;
+    const uint64_t tiles = ((N >> logR) >> logC) * batch;
+    if (launch_pass(f, logR, logE, a, tiles, block, stream) != hipSuccess) return OMR_E_DEVICE;
   }
   return OMR_OK;
 }
@@ -396,6 +413,12 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// 1 (default): pass kernels run on a persistent grid; 0: one workgroup per tile.
+extern "C" omr_status omr_set_persistent(int on) {
+  set_pass_persistent(on);
+  return OMR_OK;
+}
+
 extern "C" omr_status omr_ctx_create(int device, omr_ctx **out) {
   if (!out) return OMR_E_NULL;
   *out = nullptr;
@@ -420,6 +443,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
     (void)hipFree(kv.second.lo);
     (void)hipFree(kv.second.hi);
     (void)hipFree(kv.second.root);
+    if (kv.second.pow2_sh) (void)hipFree(kv.second.pow2_sh);
   }
   for (auto &b : ctx->scratch) free_buf(b);
   free_buf(ctx->io[0]);

@@ -36,14 +36,21 @@ inline int pass_log_cols(int logN, int logR, int logE, int esize) {
   return lc < 0 ? 0 : lc;
 }
 
-// Launches one Stockham pass; returns hipSuccess or the launch error.
-hipError_t launch_pass(int field, int logR, int logE, const PassArgs &a, dim3 grid, dim3 block,
-                       hipStream_t stream);
+// Persistent grid of a pass kernel: resident workgroups per CU x CUs (cached
+// per kernel and block size), at most one workgroup per tile; with the
+// persistent mode off, one workgroup per tile.
+unsigned pass_grid(const void *kernel, unsigned block, uint64_t total_tiles);
+void set_pass_persistent(int on);
+
+// Launches one Stockham pass over `total_tiles` tiles (a.nvec vectors);
+// returns hipSuccess or the launch error.
+hipError_t launch_pass(int field, int logR, int logE, const PassArgs &a, uint64_t total_tiles,
+                       dim3 block, hipStream_t stream);
 
 // Per-field entry points (one translation unit each).
-hipError_t launch_pass_babybear(int logR, int logE, const PassArgs &a, dim3 g, dim3 b, hipStream_t s);
-hipError_t launch_pass_koalabear(int logR, int logE, const PassArgs &a, dim3 g, dim3 b, hipStream_t s);
-hipError_t launch_pass_goldilocks(int logR, int logE, const PassArgs &a, dim3 g, dim3 b, hipStream_t s);
-hipError_t launch_pass_teddybear(int logR, int logE, const PassArgs &a, dim3 g, dim3 b, hipStream_t s);
+hipError_t launch_pass_babybear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
+hipError_t launch_pass_koalabear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
+hipError_t launch_pass_goldilocks(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
+hipError_t launch_pass_teddybear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
 
 }  // namespace omr

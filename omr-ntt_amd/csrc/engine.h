@@ -30,7 +30,7 @@ struct PassArgs {
   const void *post_lo;  // optional output-position multiplier, same two-level layout
   const void *post_hi;
   uint32_t *nonzero_count;  // if set: outputs at index >= store_limit that are nonzero are counted
-  uint64_t vec_stride_in;   // elements between consecutive input vectors (blockIdx.y)
+  uint64_t vec_stride_in;   // elements between consecutive input vectors
   uint64_t vec_stride_out;  // elements between consecutive output vectors
   uint64_t scale_w;         // multiplier applied to every output when kScale
   uint64_t store_limit;     // only outputs with index < store_limit are stored
@@ -38,6 +38,9 @@ struct PassArgs {
   uint32_t flags;     // kTwiddle | kScale | kZeroTail
   uint32_t logValid;  // with kZeroTail: input positions >= 2^logValid read as zero
   uint32_t pow2_unit, log_m;  // fields with power-of-2 roots: 2^pow2_unit has order 2^log_m
+  uint32_t nvec;              // vectors (consecutive, vec_stride_in/out apart)
+  const uint32_t *pow2_sh;    // shift amounts of root_M^m (fields with power-of-2 roots)
+  uint32_t log_rmax;          // root table holds root_(2^log_rmax)^m
 };
 enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u };
 
@@ -145,7 +148,8 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
                                            const typename F::T (&wE)[(1 << LOGE) / 2 + 1],
                                            typename F::T *lds, const typename F::T *root,
                                            uint32_t t, uint32_t c, uint32_t logC,
-                                           uint32_t root_shift) {
+                                           uint32_t root_shift, const uint32_t *pow2_sh,
+                                           uint32_t log_m) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   if constexpr (ST < K::NSTAGE) {
@@ -167,15 +171,59 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
         const uint32_t row = ps * EI + ((uint32_t)kk << LOGS) + jj;
         T v = lds[lpad((row << logC) + c)];
         const uint32_t e = (uint32_t)kk * ps;
-        if (kk != 0 && e != 0) v = F::mulw(v, root[e << root_shift]);
+        if (kk != 0 && e != 0) {
+          if constexpr (F::kPow2Roots && LOGR <= 6)  // w_R^e is a power of 2
+            v = F::mul_pow2(v, pow2_sh[e << (log_m - LOGR)]);
+          else
+            v = F::mulw(v, root[e << root_shift]);
+        }
         x[g * EI + kk] = v;
       }
     }
     GroupDft<F, LOGE, LOGEI, K::E / EI>::run(x, wE);
-    tile_stage<F, LOGR, LOGE, ST + 1>(x, wE, lds, root, t, c, logC, root_shift);
+    tile_stage<F, LOGR, LOGE, ST + 1>(x, wE, lds, root, t, c, logC, root_shift, pow2_sh, log_m);
   }
 }
 
+// Raw loads of one tile into registers (no arithmetic), so that they can be
+// issued one tile ahead.  Direct path (s >= C): the thread's E rows of its
+// column.  Staged path (s < C): the thread's E elements tid + m*nthr of the
+// tile's contiguous C*R block, transposed through LDS later.
+template <class F, int LOGR, int LOGE>
+__device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const PassArgs &a,
+                                          uint64_t gt, uint32_t tid, uint32_t c, uint32_t t,
+                                          bool staged) {
+  using K = PassKernel<F, LOGR, LOGE>;
+  using T = typename F::T;
+  const uint32_t logC = a.logC, logS = a.logS;
+  const uint32_t logTiles = a.logN - LOGR - logC;
+  const uint64_t vec = gt >> logTiles, tile = gt & ((1ull << logTiles) - 1);
+  const T *__restrict__ src = (const T *)a.src + vec * a.vec_stride_in;
+  const uint64_t valid = (a.flags & kZeroTail) ? (1ull << a.logValid) : ~0ull;
+  if (!staged) {
+    const uint64_t q = (tile << logC) + c;
+    const uint64_t S = 1ull << logS;
+    const uint64_t pos0 = (((q >> logS) << LOGR) << logS) + (q & (S - 1));
+#pragma unroll
+    for (int m = 0; m < K::E; m++) {
+      const uint64_t pos = pos0 + ((uint64_t)(t + m * K::RE) << logS);
+      xn[m] = (pos < valid) ? src[pos] : (T)0;
+    }
+  } else {
+    const uint64_t blk0 = tile << (logC + LOGR);
+    const uint32_t nthr = (1u << logC) * K::RE;
+#pragma unroll
+    for (int m = 0; m < K::E; m++) {
+      const uint64_t pos = blk0 + tid + (uint64_t)m * nthr;
+      xn[m] = (pos < valid) ? src[pos] : (T)0;
+    }
+  }
+}
+
+// Persistent, software-pipelined pass: workgroup b handles tiles
+// b, b + G, b + 2G, ... (G = gridDim.x) of all a.nvec vectors and issues the
+// loads of its next tile before computing the current one, so HBM traffic
+// overlaps the butterflies.
 template <class F, int LOGR, int LOGE>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, PassKernel<F, LOGR, LOGE>::MAXT)))
 k_stockham_pass(PassArgs a) {
@@ -190,93 +238,108 @@ k_stockham_pass(PassArgs a) {
   const uint32_t c = tid & (C - 1), t = tid >> logC;
   const uint32_t logS = a.logS;
   const uint64_t S = 1ull << logS;
-  const T *__restrict__ src = (const T *)a.src + (uint64_t)blockIdx.y * a.vec_stride_in;
-  T *__restrict__ dst = (T *)a.dst + (uint64_t)blockIdx.y * a.vec_stride_out;
+  const bool staged = S < C;
   const T *__restrict__ root = (const T *)a.root;
-
-  const uint64_t q = ((uint64_t)blockIdx.x << logC) + c;
-  const uint64_t p = q >> logS, j = q & (S - 1);
+  const uint32_t logTiles = a.logN - LOGR - logC;
+  const uint64_t total = (uint64_t)a.nvec << logTiles;
+  const uint32_t nthr = C * RE;
 
   T wE[E / 2 + 1];
   load_consts<F, LOGE>(wE, root, RE, a.root_shift, a.pow2_unit, a.log_m);
 
-  // input position of row k in this column: (p R + k) s + j
-  const uint64_t pos0 = ((p << LOGR) << logS) + j;
-  const bool zero_tail = (a.flags & kZeroTail) != 0;
-  const uint64_t valid = zero_tail ? (1ull << a.logValid) : ~0ull;
+  uint64_t gt = blockIdx.x;
+  if (gt >= total) return;
+  T xn[E];
+  load_tile<F, LOGR, LOGE>(xn, a, gt, tid, c, t, staged);
 
-  T x[E];
-  if (S >= C) {
+  for (; gt < total; gt += gridDim.x) {
+    T x[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) {
-      const uint64_t pos = pos0 + ((uint64_t)(t + m * RE) << logS);
-      x[m] = (pos < valid) ? src[pos] : (T)0;
-    }
-  } else {
-    // The tile is the contiguous block [q0 R, (q0 + C) R): element
-    // i = (pp R + k) S + jj belongs to column pp S + jj, row k.
-    const uint64_t blk0 = (uint64_t)blockIdx.x << (logC + LOGR);
-    const uint32_t total = C << LOGR, nthr = C * RE;
-    for (uint32_t i = tid; i < total; i += nthr) {
-      const uint32_t jj = i & (uint32_t)(S - 1);
-      const uint32_t k = (i >> logS) & (R - 1);
-      const uint32_t pp = i >> (logS + LOGR);
-      const uint64_t pos = blk0 + i;
-      lds[lpad((k << logC) + (pp << logS) + jj)] = (pos < valid) ? src[pos] : (T)0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < E; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
-  }
+    for (int m = 0; m < E; m++) x[m] = xn[m];
+    const uint64_t nxt = gt + gridDim.x;
+    if (nxt < total) load_tile<F, LOGR, LOGE>(xn, a, nxt, tid, c, t, staged);
 
-  if (a.pre_lo) {  // h^pos on input position pos (coset shift)
-    const T *plo = (const T *)a.pre_lo, *phi = (const T *)a.pre_hi;
-    const uint64_t lomask = (1ull << a.lo_bits) - 1;
-#pragma unroll
-    for (int m = 0; m < E; m++) {
-      const uint64_t pos = pos0 + ((uint64_t)(t + m * RE) << logS);
-      x[m] = F::mulw(x[m], F::wmul(plo[pos & lomask], phi[pos >> a.lo_bits]));
-    }
-  }
+    const uint64_t vec = gt >> logTiles, tile = gt & ((1ull << logTiles) - 1);
+    T *__restrict__ dst = (T *)a.dst + vec * a.vec_stride_out;
+    const uint64_t q = (tile << logC) + c;
+    const uint64_t p = q >> logS, j = q & (S - 1);
+    const uint64_t pos0 = ((p << LOGR) << logS) + j;
 
-  if (a.flags & kTwiddle) {  // w_N^(k p s) on input row k
-    const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
-    const uint64_t ps = p << logS;
-    const uint64_t lomask = (1ull << a.lo_bits) - 1;
+    if (staged) {
+      // element i = (pp R + k) S + jj of the block is column pp S + jj, row k
+      __syncthreads();
 #pragma unroll
-    for (int m = 0; m < E; m++) {
-      const uint64_t e = (uint64_t)(t + m * RE) * ps;
-      if (e) x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]));
-    }
-  }
-
-  // in-tile stage 0: stride R/E, one group per thread, no in-tile twiddle
-  dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0>(x, wE);
-  tile_stage<F, LOGR, LOGE, 1>(x, wE, lds, root, t, c, logC, a.root_shift);
-
-  // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
-  constexpr int EL = 1 << K::LOGLAST;
-  const uint32_t logNR = a.logN - LOGR;
-  const bool scale = (a.flags & kScale) != 0;
-  const T sw = (T)a.scale_w;
-  uint32_t nonzero = 0;
-#pragma unroll
-  for (int g = 0; g < E / EL; g++) {
-#pragma unroll
-    for (int kk = 0; kk < EL; kk++) {
-      const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
-      T v = x[g * EL + kk];
-      if (scale) v = F::mulw(v, sw);
-      const uint64_t oidx = ((uint64_t)krow << logNR) + q;
-      if (a.post_lo) {
-        const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
-        v = F::mulw(v, F::wmul(plo[oidx & ((1ull << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
+      for (int m = 0; m < E; m++) {
+        const uint32_t i = tid + (uint32_t)m * nthr;
+        const uint32_t jj = i & (uint32_t)(S - 1);
+        const uint32_t k = (i >> logS) & (R - 1);
+        const uint32_t pp = i >> (logS + LOGR);
+        lds[lpad((k << logC) + (pp << logS) + jj)] = x[m];
       }
-      if (oidx < a.store_limit) dst[oidx] = v;
-      else nonzero += (v != 0);
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < E; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
     }
+
+    if (a.pre_lo) {  // h^pos on input position pos (coset shift)
+      const T *plo = (const T *)a.pre_lo, *phi = (const T *)a.pre_hi;
+      const uint64_t lomask = (1ull << a.lo_bits) - 1;
+#pragma unroll
+      for (int m = 0; m < E; m++) {
+        const uint64_t pos = pos0 + ((uint64_t)(t + m * RE) << logS);
+        x[m] = F::mulw(x[m], F::wmul(plo[pos & lomask], phi[pos >> a.lo_bits]));
+      }
+    }
+
+    if (a.flags & kTwiddle) {  // w_N^(k p s) = w_(N/s)^(k p) on input row k
+      const uint32_t logNs = a.logN - logS;
+      if (logNs <= a.log_rmax) {  // one lookup in the subgroup table
+        const uint32_t sh = a.log_rmax - logNs;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+          const uint32_t e = (t + m * RE) * (uint32_t)p;
+          if (e) x[m] = F::mulw(x[m], root[e << sh]);
+        }
+      } else {  // two-level table
+        const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
+        const uint64_t ps = p << logS;
+        const uint64_t lomask = (1ull << a.lo_bits) - 1;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+          const uint64_t e = (uint64_t)(t + m * RE) * ps;
+          if (e) x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]));
+        }
+      }
+    }
+
+    // in-tile stage 0: stride R/E, one group per thread, no in-tile twiddle
+    dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0>(x, wE);
+    tile_stage<F, LOGR, LOGE, 1>(x, wE, lds, root, t, c, logC, a.root_shift, a.pow2_sh, a.log_m);
+
+    // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
+    constexpr int EL = 1 << K::LOGLAST;
+    const uint32_t logNR = a.logN - LOGR;
+    const bool scale = (a.flags & kScale) != 0;
+    const T sw = (T)a.scale_w;
+    uint32_t nonzero = 0;
+#pragma unroll
+    for (int g = 0; g < E / EL; g++) {
+#pragma unroll
+      for (int kk = 0; kk < EL; kk++) {
+        const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
+        T v = x[g * EL + kk];
+        if (scale) v = F::mulw(v, sw);
+        const uint64_t oidx = ((uint64_t)krow << logNR) + q;
+        if (a.post_lo) {
+          const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
+          v = F::mulw(v, F::wmul(plo[oidx & ((1ull << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
+        }
+        if (oidx < a.store_limit) dst[oidx] = v;
+        else nonzero += (v != 0);
+      }
+    }
+    if (a.nonzero_count && nonzero) atomicAdd(a.nonzero_count, nonzero);
   }
-  if (a.nonzero_count && nonzero) atomicAdd(a.nonzero_count, nonzero);
 }
 
 }  // namespace omr

@@ -23,6 +23,7 @@ struct TwTables {
   void *root = nullptr;  // root^(m 2^(logN - log_rmax)), m < 2^log_rmax
   uint32_t lo_bits = 0;
   uint32_t log_rmax = 0;
+  void *pow2_sh = nullptr;  // Goldilocks: shift amount of root_M^m, m < M (uint32)
   uint32_t pow2_unit = 0;  // Goldilocks: 2^pow2_unit = root^(2^logN / M), M = 2^log_m = min(2^logN, 64)
   uint32_t log_m = 0;
 };
@@ -84,6 +85,7 @@ void set_tuned_schedule(int f, uint32_t logN, const Schedule &s);
 bool tuned_partial(int f, uint32_t logN, uint32_t logOut, int &method, uint32_t &log_n1);
 
 // Schedule-class vocabulary shared by the autotuner and the cost model.
+constexpr int kLogSubgroup = 18;  // size of the cached subgroup root table (2^18 entries)
 constexpr int kFullClasses = 30;
 constexpr int kNumClasses = 36;
 bool class_schedule(int f, int cls, uint32_t logN, Schedule &s);

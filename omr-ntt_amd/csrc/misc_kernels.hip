@@ -3,6 +3,10 @@
 #include "internal.h"
 #include "misc.h"
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 namespace omr {
 
 #define OMR_FIELD_SWITCH(f, BODY)          \
@@ -120,15 +124,45 @@ omr_status launch_field_op(int f, int op, const void *a, const void *b, void *ou
   return last_launch();
 }
 
-hipError_t launch_pass(int field, int logR, int logE, const PassArgs &a, dim3 grid, dim3 block,
+hipError_t launch_pass(int field, int logR, int logE, const PassArgs &a, uint64_t total_tiles, dim3 block,
                        hipStream_t stream) {
   switch (field) {
-    case kBabyBear: return launch_pass_babybear(logR, logE, a, grid, block, stream);
-    case kKoalaBear: return launch_pass_koalabear(logR, logE, a, grid, block, stream);
-    case kGoldilocks: return launch_pass_goldilocks(logR, logE, a, grid, block, stream);
-    case kTeddyBear: return launch_pass_teddybear(logR, logE, a, grid, block, stream);
+    case kBabyBear: return launch_pass_babybear(logR, logE, a, total_tiles, block, stream);
+    case kKoalaBear: return launch_pass_koalabear(logR, logE, a, total_tiles, block, stream);
+    case kGoldilocks: return launch_pass_goldilocks(logR, logE, a, total_tiles, block, stream);
+    case kTeddyBear: return launch_pass_teddybear(logR, logE, a, total_tiles, block, stream);
   }
   return hipErrorInvalidValue;
+}
+
+static std::mutex g_grid_mu;
+static std::map<std::pair<const void *, unsigned>, unsigned> g_grid_cache;
+static int g_persistent = 1;
+
+void set_pass_persistent(int on) { g_persistent = on ? 1 : 0; }
+
+unsigned pass_grid(const void *kernel, unsigned block, uint64_t total_tiles) {
+  if (!g_persistent) return (unsigned)std::min<uint64_t>(total_tiles, 0x7FFFFFFFull);
+  unsigned g;
+  {
+    std::lock_guard<std::mutex> lk(g_grid_mu);
+    auto it = g_grid_cache.find({kernel, block});
+    if (it != g_grid_cache.end()) {
+      g = it->second;
+    } else {
+      int dev = 0, occ = 0;
+      hipDeviceProp_t prop;
+      (void)hipGetDevice(&dev);
+      int cus = hipGetDeviceProperties(&prop, dev) == hipSuccess ? prop.multiProcessorCount : 256;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, (int)block, 0) != hipSuccess || occ < 1) {
+        (void)hipGetLastError();
+        occ = 1;
+      }
+      g = (unsigned)(occ * cus);
+      g_grid_cache[{kernel, block}] = g;
+    }
+  }
+  return (unsigned)std::min<uint64_t>(total_tiles, g);
 }
 
 }  // namespace omr

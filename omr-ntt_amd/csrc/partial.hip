@@ -42,15 +42,17 @@ static omr_status first_pass(omr_ctx *ctx, int f, const void *in, void *out, uin
   a.root_shift = tab->log_rmax - logR;
   a.pow2_unit = tab->pow2_unit;
   a.log_m = tab->log_m;
+  a.pow2_sh = (const uint32_t *)tab->pow2_sh;
+  a.log_rmax = tab->log_rmax;
   a.logN = logN;
   a.logS = logN - logR;
   a.logC = logC;
   a.vec_stride_in = a.vec_stride_out = 0;
   a.store_limit = store_limit;
+  a.nvec = 1;
   const uint64_t N = 1ull << logN;
   dim3 block((1u << logC) << (logR - logE));
-  dim3 grid((unsigned)((N >> logR) >> logC), 1);
-  return launch_pass(f, logR, logE, a, grid, block, s) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+  return launch_pass(f, logR, logE, a, (N >> logR) >> logC, block, s) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
 }
 
 omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
