@@ -143,10 +143,11 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
   TwTables t{};
   t.lo_bits = std::min<uint32_t>(logN, 12);
   const uint64_t nlo = 1ull << t.lo_bits, nhi = 1ull << (logN - t.lo_bits);
-  // subgroup table: root^(m 2^(logN - log_rmax)), m < 2^log_rmax, serves the
-  // in-tile roots of every pass radix and single-lookup inter-pass twiddles
-  // for passes with N/s <= 2^log_rmax
-  t.log_rmax = std::min<uint32_t>(logN, kLogSubgroup);
+  // root table: root^(m 2^(logN - log_rmax)), m < 2^log_rmax (the largest pass
+  // radix): in-tile roots of every pass radix, and single-lookup inter-pass
+  // twiddles for passes with N/s <= 2^log_rmax.  Kept compact: strided
+  // lookups into a larger table miss the L1 cache.
+  t.log_rmax = std::min<uint32_t>(logN, (uint32_t)max_log_radix(es));
   const uint64_t nroot = 1ull << t.log_rmax;
   std::vector<uint64_t> lo(nlo), hi(nhi), rt(nroot);
   uint64_t cur = 1;
@@ -191,10 +192,11 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
         t.pow2_unit = k;
         break;
       }
-    if (t.pow2_unit >= 192) return OMR_E_OMEGA;  // unreachable for a root of order <= 64
+    // tables of a value that is not a root of unity (coset shifts) have none
+    if (t.pow2_unit >= 192) t.pow2_unit = 0, t.log_m = 0;
   }
   std::vector<uint64_t> sh;
-  if (f == kGoldilocks) {  // shift amounts of the powers of the root of order M = 2^log_m
+  if (f == kGoldilocks && t.log_m > 0) {  // shift amounts of the powers of the root of order M = 2^log_m
     sh.resize(1ull << t.log_m);
     for (size_t m = 0; m < sh.size(); m++) sh[m] = (t.pow2_unit * m) % 192;
   }

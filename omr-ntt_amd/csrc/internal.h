@@ -85,7 +85,6 @@ void set_tuned_schedule(int f, uint32_t logN, const Schedule &s);
 bool tuned_partial(int f, uint32_t logN, uint32_t logOut, int &method, uint32_t &log_n1);
 
 // Schedule-class vocabulary shared by the autotuner and the cost model.
-constexpr int kLogSubgroup = 18;  // size of the cached subgroup root table (2^18 entries)
 constexpr int kFullClasses = 30;
 constexpr int kNumClasses = 36;
 bool class_schedule(int f, int cls, uint32_t logN, Schedule &s);
