@@ -184,10 +184,6 @@ omr_status omr_batch_execute_graph(omr_batch_plan *plan, const void *d_in, void 
 /* Baseline form: the single-vector transform launched once per vector. */
 omr_status omr_batch_execute_plain(omr_batch_plan *plan, const void *d_in, void *d_out,
                                    void *stream);
-/* Pass kernels run on a persistent, software-pipelined grid (on != 0, the
- * default) or with one workgroup per tile (on == 0). */
-omr_status omr_set_persistent(int on);
-
 /* Largest pass radix (2^4 .. 2^12) the batch planner gives multi-pass vectors
  * of later plans (default 2^9). */
 omr_status omr_batch_set_max_log_radix(int log_radix);

@@ -415,12 +415,6 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-// 1 (default): pass kernels run on a persistent grid; 0: one workgroup per tile.
-extern "C" omr_status omr_set_persistent(int on) {
-  set_pass_persistent(on);
-  return OMR_OK;
-}
-
 extern "C" omr_status omr_ctx_create(int device, omr_ctx **out) {
   if (!out) return OMR_E_NULL;
   *out = nullptr;

@@ -36,11 +36,8 @@ inline int pass_log_cols(int logN, int logR, int logE, int esize) {
   return lc < 0 ? 0 : lc;
 }
 
-// Persistent grid of a pass kernel: resident workgroups per CU x CUs (cached
-// per kernel and block size), at most one workgroup per tile; with the
-// persistent mode off, one workgroup per tile.
+// Grid of a pass kernel: one workgroup per tile.
 unsigned pass_grid(const void *kernel, unsigned block, uint64_t total_tiles);
-void set_pass_persistent(int on);
 
 // Launches one Stockham pass over `total_tiles` tiles (a.nvec vectors);
 // returns hipSuccess or the launch error.

@@ -185,8 +185,7 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
   }
 }
 
-// Raw loads of one tile into registers (no arithmetic), so that they can be
-// issued one tile ahead.  Direct path (s >= C): the thread's E rows of its
+// Raw loads of one tile into registers (no arithmetic).  Direct path (s >= C): the thread's E rows of its
 // column.  Staged path (s < C): the thread's E elements tid + m*nthr of the
 // tile's contiguous C*R block, transposed through LDS later.
 template <class F, int LOGR, int LOGE>
@@ -220,10 +219,10 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
   }
 }
 
-// Persistent, software-pipelined pass: workgroup b handles tiles
-// b, b + G, b + 2G, ... (G = gridDim.x) of all a.nvec vectors and issues the
-// loads of its next tile before computing the current one, so HBM traffic
-// overlaps the butterflies.
+// Pass kernel: workgroup b owns tile b of the a.nvec vectors' tiles.  (A
+// grid-stride loop over several tiles per workgroup made the compiler keep
+// per-tile invariants live across iterations: 2-3x the VGPRs and a quarter
+// of the occupancy, measured slower.)
 template <class F, int LOGR, int LOGE>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, PassKernel<F, LOGR, LOGE>::MAXT)))
 k_stockham_pass(PassArgs a) {
@@ -247,17 +246,11 @@ k_stockham_pass(PassArgs a) {
   T wE[E / 2 + 1];
   load_consts<F, LOGE>(wE, root, RE, a.root_shift, a.pow2_unit, a.log_m);
 
-  uint64_t gt = blockIdx.x;
-  if (gt >= total) return;
-  T xn[E];
-  load_tile<F, LOGR, LOGE>(xn, a, gt, tid, c, t, staged);
-
-  for (; gt < total; gt += gridDim.x) {
+  {
+    const uint64_t gt = blockIdx.x;
+    if (gt >= total) return;  // grid = total tiles
     T x[E];
-#pragma unroll
-    for (int m = 0; m < E; m++) x[m] = xn[m];
-    const uint64_t nxt = gt + gridDim.x;
-    if (nxt < total) load_tile<F, LOGR, LOGE>(xn, a, nxt, tid, c, t, staged);
+    load_tile<F, LOGR, LOGE>(x, a, gt, tid, c, t, staged);
 
     const uint64_t vec = gt >> logTiles, tile = gt & ((1ull << logTiles) - 1);
     T *__restrict__ dst = (T *)a.dst + vec * a.vec_stride_out;

@@ -135,34 +135,8 @@ hipError_t launch_pass(int field, int logR, int logE, const PassArgs &a, uint64_
   return hipErrorInvalidValue;
 }
 
-static std::mutex g_grid_mu;
-static std::map<std::pair<const void *, unsigned>, unsigned> g_grid_cache;
-static int g_persistent = 1;
-
-void set_pass_persistent(int on) { g_persistent = on ? 1 : 0; }
-
-unsigned pass_grid(const void *kernel, unsigned block, uint64_t total_tiles) {
-  if (!g_persistent) return (unsigned)std::min<uint64_t>(total_tiles, 0x7FFFFFFFull);
-  unsigned g;
-  {
-    std::lock_guard<std::mutex> lk(g_grid_mu);
-    auto it = g_grid_cache.find({kernel, block});
-    if (it != g_grid_cache.end()) {
-      g = it->second;
-    } else {
-      int dev = 0, occ = 0;
-      hipDeviceProp_t prop;
-      (void)hipGetDevice(&dev);
-      int cus = hipGetDeviceProperties(&prop, dev) == hipSuccess ? prop.multiProcessorCount : 256;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, (int)block, 0) != hipSuccess || occ < 1) {
-        (void)hipGetLastError();
-        occ = 1;
-      }
-      g = (unsigned)(occ * cus);
-      g_grid_cache[{kernel, block}] = g;
-    }
-  }
-  return (unsigned)std::min<uint64_t>(total_tiles, g);
+unsigned pass_grid(const void *, unsigned, uint64_t total_tiles) {
+  return (unsigned)std::min<uint64_t>(total_tiles, 0x7FFFFFFFull);
 }
 
 }  // namespace omr

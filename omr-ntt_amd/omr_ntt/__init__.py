@@ -97,7 +97,6 @@ def lib():
             "omr_batch_execute_graph": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_execute_plain": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_set_max_log_radix": (i32, [i32]),
-            "omr_set_persistent": (i32, [i32]),
             "omr_num_schedule_classes": (i32, []),
             "omr_schedule_class": (i32, [i32, u64, u64, i32, Pi, Pi, Pi, Pi, ctypes.POINTER(u32)]),
             "omr_default_class": (i32, [i32, u64, u64]),
@@ -317,12 +316,6 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
-
-
-def set_persistent(on):
-    """Pass kernels on a persistent, software-pipelined grid (default) or one
-    workgroup per tile."""
-    _check(lib().omr_set_persistent(1 if on else 0))
 
 
 def batch_set_max_log_radix(r):
