@@ -134,7 +134,7 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
       (void)hipFree(kv.second.lo);
       (void)hipFree(kv.second.hi);
       (void)hipFree(kv.second.root);
-      if (kv.second.pow2_sh) (void)hipFree(kv.second.pow2_sh);
+      if (kv.second.mid) (void)hipFree(kv.second.mid);
     }
     ctx->tables.clear();
   }
@@ -195,16 +195,19 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
     // tables of a value that is not a root of unity (coset shifts) have none
     if (t.pow2_unit >= 192) t.pow2_unit = 0, t.log_m = 0;
   }
-  std::vector<uint64_t> sh;
-  if (f == kGoldilocks && t.log_m > 0) {  // shift amounts of the powers of the root of order M = 2^log_m
-    sh.resize(1ull << t.log_m);
-    for (size_t m = 0; m < sh.size(); m++) sh[m] = (t.pow2_unit * m) % 192;
-  }
   omr_status st;
-  if (!sh.empty()) {
-    std::vector<uint32_t> sh32(sh.begin(), sh.end());
-    if (hipMalloc(&t.pow2_sh, sh32.size() * 4) != hipSuccess) return OMR_E_ALLOC;
-    if (hipMemcpy(t.pow2_sh, sh32.data(), sh32.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
+  t.log_mid = std::min<uint32_t>(logN, kLogMid);
+  if (t.log_mid > t.log_rmax) {  // mid table root^(j N/2^log_mid), j < 2^log_mid
+    std::vector<uint64_t> mid(1ull << t.log_mid);
+    const uint64_t step = fpow_host(f, root, 1ull << (logN - t.log_mid));
+    cur = 1;
+    for (auto &v : mid) {
+      v = to_w_host(f, cur);
+      cur = fmul_host(f, cur, step);
+    }
+    if ((st = upload(mid, &t.mid)) != OMR_OK) return st;
+  } else {
+    t.log_mid = t.log_rmax;
   }
   if ((st = upload(lo, &t.lo)) != OMR_OK) return st;
   if ((st = upload(hi, &t.hi)) != OMR_OK) return st;
@@ -373,10 +376,9 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.root = tab->root;
     a.lo_bits = tab->lo_bits;
     a.root_shift = tab->log_rmax - logR;
-    a.pow2_unit = tab->pow2_unit;
-    a.log_m = tab->log_m;
-    a.pow2_sh = (const uint32_t *)tab->pow2_sh;
+    a.tw_mid = tab->mid;
     a.log_rmax = tab->log_rmax;
+    a.log_mid = tab->log_mid;
     a.logN = logN;
     a.logS = logN - consumed;
     a.logC = logC;
@@ -405,7 +407,7 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.nvec = (uint32_t)batch;
     dim3 block((1u << logC) << (logR - logE));
     const uint64_t tiles = ((N >> logR) >> logC) * batch;
-    if (launch_pass(f, logR, logE, a, tiles, block, stream) != hipSuccess) return OMR_E_DEVICE;
+    if (launch_pass(f, logR, logE, tab->unit(), a, tiles, block, stream) != hipSuccess) return OMR_E_DEVICE;
   }
   return OMR_OK;
 }
@@ -439,7 +441,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
     (void)hipFree(kv.second.lo);
     (void)hipFree(kv.second.hi);
     (void)hipFree(kv.second.root);
-    if (kv.second.pow2_sh) (void)hipFree(kv.second.pow2_sh);
+    if (kv.second.mid) (void)hipFree(kv.second.mid);
   }
   for (auto &b : ctx->scratch) free_buf(b);
   free_buf(ctx->io[0]);

@@ -157,8 +157,6 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
       bt.root = tab->root;
       bt.lo_bits = tab->lo_bits;
       bt.log_rmax = tab->log_rmax;
-      bt.pow2_unit = tab->pow2_unit;
-      bt.log_m = tab->log_m;
       tset_of[key] = (uint16_t)tsets.size();
       tsets.push_back(bt);
     }

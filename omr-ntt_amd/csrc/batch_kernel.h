@@ -33,7 +33,6 @@ struct BatchTables {  // tables of one root of order 2^logN, multiplier form
   const void *lo, *hi, *root;
   uint64_t scale_w;
   uint32_t lo_bits, log_rmax;
-  uint32_t pow2_unit, log_m;  // power-of-2 roots (see load_consts)
 };
 
 struct BatchBases {
@@ -93,7 +92,7 @@ k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchV
     const uint32_t root_shift = tb.log_rmax - logR;
 
     T wE[9];
-    load_consts<F, 4>(wE, root, RE, root_shift, tb.pow2_unit, tb.log_m);
+    load_consts<F, 4>(wE, root, RE, root_shift);
 
     T x[16];
     if (S >= (1ull << logCv)) {

@@ -41,13 +41,18 @@ unsigned pass_grid(const void *kernel, unsigned block, uint64_t total_tiles);
 
 // Launches one Stockham pass over `total_tiles` tiles (a.nvec vectors);
 // returns hipSuccess or the launch error.
-hipError_t launch_pass(int field, int logR, int logE, const PassArgs &a, uint64_t total_tiles,
+// unit: 0, or (Goldilocks, N >= 64) 39 / 153 when the 64th root of the
+// transform's root is 2^39 / 2^153 -- register-DFT constants become
+// compile-time shifts.
+hipError_t launch_pass(int field, int logR, int logE, int unit, const PassArgs &a, uint64_t total_tiles,
                        dim3 block, hipStream_t stream);
 
 // Per-field entry points (one translation unit each).
 hipError_t launch_pass_babybear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
 hipError_t launch_pass_koalabear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
-hipError_t launch_pass_goldilocks(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
+hipError_t launch_pass_goldilocks_u0(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
+hipError_t launch_pass_goldilocks_u39(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
+hipError_t launch_pass_goldilocks_u153(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
 hipError_t launch_pass_teddybear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
 
 }  // namespace omr

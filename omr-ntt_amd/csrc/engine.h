@@ -37,10 +37,10 @@ struct PassArgs {
   uint32_t logN, logS, logC, lo_bits, root_shift;
   uint32_t flags;     // kTwiddle | kScale | kZeroTail
   uint32_t logValid;  // with kZeroTail: input positions >= 2^logValid read as zero
-  uint32_t pow2_unit, log_m;  // fields with power-of-2 roots: 2^pow2_unit has order 2^log_m
   uint32_t nvec;              // vectors (consecutive, vec_stride_in/out apart)
-  const uint32_t *pow2_sh;    // shift amounts of root_M^m (fields with power-of-2 roots)
+  const void *tw_mid;         // w_N^(j N/2^log_mid), j < 2^log_mid (single-lookup twiddles)
   uint32_t log_rmax;          // root table holds root_(2^log_rmax)^m
+  uint32_t log_mid;
 };
 enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u };
 
@@ -53,23 +53,24 @@ __device__ __forceinline__ constexpr int brev_c(int i) {
 
 // Multiply by a register-DFT constant: wE holds multiplier-form roots, or
 // (fields whose small roots are powers of 2) the shift amounts of 2^k.
-template <class F>
-__device__ __forceinline__ typename F::T const_mul(typename F::T d, typename F::T c) {
-  if constexpr (F::kPow2Roots) return F::mul_pow2(d, (uint32_t)c);
+// U != 0 (fields with power-of-2 roots, N >= 64): w_64 = 2^U, so w_E^idx =
+// 2^((U idx 64/E) mod 192) is a compile-time shift once the DFT loops are
+// unrolled.  U == 0: c is the multiplier-form constant.
+template <class F, int U, int LOGE>
+__device__ __forceinline__ typename F::T const_mul(typename F::T d, typename F::T c, int idx) {
+  if constexpr (U != 0) return F::mul_pow2(d, (uint32_t)((U * idx * (64 >> LOGE)) % 192));
   else return F::mulw(d, c);
 }
 
-// Register-DFT constants w_E^k, k <= E/2: multiplier-form values loaded from
-// the root table, or shift amounts (pow2_unit * k * (M/E)) mod 192 where
-// 2^pow2_unit is the root of order M = 2^log_m (M = min(N, 64)).
-template <class F, int LOGE>
+// Register-DFT constants w_E^k, k <= E/2: multiplier-form values from the
+// root table (unused when the constants are compile-time shifts, U != 0).
+template <class F, int LOGE, int U = 0>
 __device__ __forceinline__ void load_consts(typename F::T (&wE)[(1 << LOGE) / 2 + 1], const typename F::T *root,
-                                            uint32_t RE, uint32_t root_shift, uint32_t pow2_unit,
-                                            uint32_t log_m) {
+                                            uint32_t RE, uint32_t root_shift) {
 #pragma unroll
   for (int k = 0; k <= (1 << LOGE) / 2; k++) {
-    if constexpr (F::kPow2Roots)
-      wE[k] = (typename F::T)((pow2_unit * ((uint32_t)k << (log_m - LOGE))) % 192u);
+    if constexpr (U != 0)
+      wE[k] = 0;
     else
       wE[k] = root[((uint32_t)k * RE) << root_shift];
   }
@@ -79,7 +80,7 @@ __device__ __forceinline__ void load_consts(typename F::T (&wE)[(1 << LOGE) / 2 
 // and out.  wE[k] = w_E^k for the register radix E = 2^LOGE; the sub-DFT
 // uses w_X = w_E^(E/X).  Radix-2 decimation in frequency followed by a
 // compile-time register permutation (free).
-template <class F, int LOGE, int LOGX, int OFF>
+template <class F, int LOGE, int LOGX, int OFF, int U = 0>
 __device__ __forceinline__ void dft_reg(typename F::T (&x)[1 << LOGE],
                                         const typename F::T (&wE)[(1 << LOGE) / 2 + 1]) {
   using T = typename F::T;
@@ -95,7 +96,7 @@ __device__ __forceinline__ void dft_reg(typename F::T (&x)[1 << LOGE],
         T u = x[OFF + b + j], v = x[OFF + b + j + half];
         x[OFF + b + j] = F::add(u, v);
         T d = F::sub(u, v);
-        x[OFF + b + j + half] = (j == 0) ? d : const_mul<F>(d, wE[j * (E / len)]);
+        x[OFF + b + j + half] = (j == 0) ? d : const_mul<F, U, LOGE>(d, wE[j * (E / len)], j * (E / len));
       }
     }
   }
@@ -107,16 +108,16 @@ __device__ __forceinline__ void dft_reg(typename F::T (&x)[1 << LOGE],
 }
 
 // E / X independent sub-DFTs of size X = 2^LOGX on consecutive register groups.
-template <class F, int LOGE, int LOGX, int G>
+template <class F, int LOGE, int LOGX, int G, int U = 0>
 struct GroupDft {
   __device__ __forceinline__ static void run(typename F::T (&x)[1 << LOGE],
                                              const typename F::T (&wE)[(1 << LOGE) / 2 + 1]) {
-    GroupDft<F, LOGE, LOGX, G - 1>::run(x, wE);
-    dft_reg<F, LOGE, LOGX, (G - 1) << LOGX>(x, wE);
+    GroupDft<F, LOGE, LOGX, G - 1, U>::run(x, wE);
+    dft_reg<F, LOGE, LOGX, (G - 1) << LOGX, U>(x, wE);
   }
 };
-template <class F, int LOGE, int LOGX>
-struct GroupDft<F, LOGE, LOGX, 0> {
+template <class F, int LOGE, int LOGX, int U>
+struct GroupDft<F, LOGE, LOGX, 0, U> {
   __device__ __forceinline__ static void run(typename F::T (&)[1 << LOGE],
                                              const typename F::T (&)[(1 << LOGE) / 2 + 1]) {}
 };
@@ -143,13 +144,12 @@ __device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
 
 // In-tile stage ST >= 1 of the R-point Stockham: reads the previous stage's
 // outputs from LDS, applies w_R^(k' p s), runs the register DFTs.
-template <class F, int LOGR, int LOGE, int ST>
+template <class F, int LOGR, int LOGE, int ST, int U = 0>
 __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
                                            const typename F::T (&wE)[(1 << LOGE) / 2 + 1],
                                            typename F::T *lds, const typename F::T *root,
                                            uint32_t t, uint32_t c, uint32_t logC,
-                                           uint32_t root_shift, const uint32_t *pow2_sh,
-                                           uint32_t log_m) {
+                                           uint32_t root_shift) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   if constexpr (ST < K::NSTAGE) {
@@ -171,17 +171,12 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
         const uint32_t row = ps * EI + ((uint32_t)kk << LOGS) + jj;
         T v = lds[lpad((row << logC) + c)];
         const uint32_t e = (uint32_t)kk * ps;
-        if (kk != 0 && e != 0) {
-          if constexpr (F::kPow2Roots && LOGR <= 6)  // w_R^e is a power of 2
-            v = F::mul_pow2(v, pow2_sh[e << (log_m - LOGR)]);
-          else
-            v = F::mulw(v, root[e << root_shift]);
-        }
+        if (kk != 0 && e != 0) v = F::mulw(v, root[e << root_shift]);
         x[g * EI + kk] = v;
       }
     }
-    GroupDft<F, LOGE, LOGEI, K::E / EI>::run(x, wE);
-    tile_stage<F, LOGR, LOGE, ST + 1>(x, wE, lds, root, t, c, logC, root_shift, pow2_sh, log_m);
+    GroupDft<F, LOGE, LOGEI, K::E / EI, U>::run(x, wE);
+    tile_stage<F, LOGR, LOGE, ST + 1, U>(x, wE, lds, root, t, c, logC, root_shift);
   }
 }
 
@@ -223,7 +218,7 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
 // grid-stride loop over several tiles per workgroup made the compiler keep
 // per-tile invariants live across iterations: 2-3x the VGPRs and a quarter
 // of the occupancy, measured slower.)
-template <class F, int LOGR, int LOGE>
+template <class F, int LOGR, int LOGE, int U>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, PassKernel<F, LOGR, LOGE>::MAXT)))
 k_stockham_pass(PassArgs a) {
   using K = PassKernel<F, LOGR, LOGE>;
@@ -244,7 +239,7 @@ k_stockham_pass(PassArgs a) {
   const uint32_t nthr = C * RE;
 
   T wE[E / 2 + 1];
-  load_consts<F, LOGE>(wE, root, RE, a.root_shift, a.pow2_unit, a.log_m);
+  load_consts<F, LOGE, U>(wE, root, RE, a.root_shift);
 
   {
     const uint64_t gt = blockIdx.x;
@@ -286,12 +281,14 @@ k_stockham_pass(PassArgs a) {
 
     if (a.flags & kTwiddle) {  // w_N^(k p s) = w_(N/s)^(k p) on input row k
       const uint32_t logNs = a.logN - logS;
-      if (logNs <= a.log_rmax) {  // one lookup in the subgroup table
-        const uint32_t sh = a.log_rmax - logNs;
+      if (logNs <= a.log_mid) {  // one lookup: w_(N/s)^e from the root or the mid table
+        const bool small = logNs <= a.log_rmax;
+        const T *tab = small ? root : (const T *)a.tw_mid;
+        const uint32_t sh = (small ? a.log_rmax : a.log_mid) - logNs;
 #pragma unroll
         for (int m = 0; m < E; m++) {
           const uint32_t e = (t + m * RE) * (uint32_t)p;
-          if (e) x[m] = F::mulw(x[m], root[e << sh]);
+          if (e) x[m] = F::mulw(x[m], tab[e << sh]);
         }
       } else {  // two-level table
         const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
@@ -306,8 +303,8 @@ k_stockham_pass(PassArgs a) {
     }
 
     // in-tile stage 0: stride R/E, one group per thread, no in-tile twiddle
-    dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0>(x, wE);
-    tile_stage<F, LOGR, LOGE, 1>(x, wE, lds, root, t, c, logC, a.root_shift, a.pow2_sh, a.log_m);
+    dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0, U>(x, wE);
+    tile_stage<F, LOGR, LOGE, 1, U>(x, wE, lds, root, t, c, logC, a.root_shift);
 
     // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
     constexpr int EL = 1 << K::LOGLAST;

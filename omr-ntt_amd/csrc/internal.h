@@ -23,9 +23,12 @@ struct TwTables {
   void *root = nullptr;  // root^(m 2^(logN - log_rmax)), m < 2^log_rmax
   uint32_t lo_bits = 0;
   uint32_t log_rmax = 0;
-  void *pow2_sh = nullptr;  // Goldilocks: shift amount of root_M^m, m < M (uint32)
+  void *mid = nullptr;     // root^(j 2^(logN - log_mid)), j < 2^log_mid: single-lookup twiddles
+  uint32_t log_mid = 0;
   uint32_t pow2_unit = 0;  // Goldilocks: 2^pow2_unit = root^(2^logN / M), M = 2^log_m = min(2^logN, 64)
   uint32_t log_m = 0;
+  // register-DFT constant unit of the pass kernels (0 = table constants)
+  int unit() const { return (log_m == 6 && (pow2_unit == 39 || pow2_unit == 153)) ? (int)pow2_unit : 0; }
 };
 
 // A transform schedule: pass radices 2^logR[i] (sum = logN), register radix
@@ -85,6 +88,7 @@ void set_tuned_schedule(int f, uint32_t logN, const Schedule &s);
 bool tuned_partial(int f, uint32_t logN, uint32_t logOut, int &method, uint32_t &log_n1);
 
 // Schedule-class vocabulary shared by the autotuner and the cost model.
+constexpr int kLogMid = 16;  // mid twiddle table: 2^16 entries
 constexpr int kFullClasses = 30;
 constexpr int kNumClasses = 36;
 bool class_schedule(int f, int cls, uint32_t logN, Schedule &s);

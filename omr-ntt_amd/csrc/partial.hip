@@ -40,10 +40,9 @@ static omr_status first_pass(omr_ctx *ctx, int f, const void *in, void *out, uin
   a.root = tab->root;
   a.lo_bits = tab->lo_bits;
   a.root_shift = tab->log_rmax - logR;
-  a.pow2_unit = tab->pow2_unit;
-  a.log_m = tab->log_m;
-  a.pow2_sh = (const uint32_t *)tab->pow2_sh;
+  a.tw_mid = tab->mid;
   a.log_rmax = tab->log_rmax;
+  a.log_mid = tab->log_mid;
   a.logN = logN;
   a.logS = logN - logR;
   a.logC = logC;
@@ -52,7 +51,8 @@ static omr_status first_pass(omr_ctx *ctx, int f, const void *in, void *out, uin
   a.nvec = 1;
   const uint64_t N = 1ull << logN;
   dim3 block((1u << logC) << (logR - logE));
-  return launch_pass(f, logR, logE, a, (N >> logR) >> logC, block, s) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+  return launch_pass(f, logR, logE, tab->unit(), a, (N >> logR) >> logC, block, s) == hipSuccess ? OMR_OK
+                                                                                                  : OMR_E_DEVICE;
 }
 
 omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
