@@ -82,6 +82,70 @@ def other_sizes(E, F, torch, np):
     return out
 
 
+def config1(E, torch, np):
+    """BabyBear 2^16 forward + inverse, latency per transform."""
+    F = E.BabyBear
+    n = 1 << 16
+    x = E.to_device(np.random.default_rng(16).integers(0, F.p, size=n, dtype=np.uint64), F)
+    y = E.empty_device(n, F)
+    fwd = time_device(lambda i: E.ntt_device(x, y, F), 200, 20, torch)
+    inv = time_device(lambda i: E.intt_device(y, x, F), 200, 20, torch)
+    log(f"bench: BabyBear 2^16 forward {fwd * 1e6:.1f} us inverse {inv * 1e6:.1f} us")
+    return {"forward_us": fwd * 1e6, "inverse_us": inv * 1e6, "schedule": E.default_schedule(F, n),
+            "elems_per_s": n / fwd}
+
+
+def config3(E, torch, np):
+    """Partial outputs of N = 2^22: prefix (out_len) and stride subset."""
+    out = {}
+    n = 1 << 22
+    for F in (E.BabyBear, E.Goldilocks):
+        x = E.to_device(np.random.default_rng(22).integers(0, F.p, size=n, dtype=np.uint64), F)
+        y = E.empty_device(n, F)
+        full = time_device(lambda i: E.ntt_device(x, y, F), 20, 3, torch)
+        rows = {"full_us": full * 1e6}
+        for lo in (8, 12, 16, 20):
+            m = 1 << lo
+            o = E.empty_device(m, F)
+            pre = time_device(lambda i: E.partial_device(x, o, F, m, mode=E.PREFIX), 20, 3, torch)
+            strd = time_device(lambda i: E.partial_device(x, o, F, m, mode=E.STRIDE), 20, 3, torch)
+            rows[f"2^{lo}"] = {"prefix_us": pre * 1e6, "stride_us": strd * 1e6}
+        out[F.name] = rows
+        log(f"bench: partial {F.name} 2^22: " + ", ".join(
+            f"{k} {v['prefix_us']:.1f}/{v['stride_us']:.1f} us" for k, v in rows.items() if k != "full_us")
+            + f" (full {full * 1e6:.1f} us)")
+    return out
+
+
+def config5(E, torch, np):
+    """256 BabyBear columns x 2^20 coefficients -> blowup 4 coset LDE
+    (n = 2^22) -> decode, with the schedule chosen by the learned cost model
+    (trained live on autotuner timings); held-out regret reported."""
+    from omr_ntt import costmodel as CM
+    shapes = CM.shape_grid(fields=[E.BabyBear, E.Goldilocks], log_ns=range(12, 23, 2), batches=(1, 16, 64),
+                           partial=False, max_elems=1 << 27)
+    res, model = CM.experiment(shapes, reps=3, seed=0, steps=3000)
+    F = E.BabyBear
+    n, cols, blow = 1 << 20, 256, 4
+    m = n * blow
+    cls, timed = E.autotune_shape(F, m, m, cols, model=model, use_model=True)
+    sched = E.schedule_class(F, m, m, cls)
+    coeffs = torch.randint(0, F.p, (n * cols,), dtype=torch.int32, device="cuda")
+    evals = torch.empty(m * cols, dtype=torch.int32, device="cuda")
+    back = torch.empty_like(coeffs)
+    nz = torch.zeros(1, dtype=torch.int32, device="cuda")
+    enc = time_device(lambda i: E.coset_lde_device(coeffs, evals, F, n, cols, blow), 5, 1, torch)
+    dec = time_device(lambda i: E.coset_decode_device(evals, back, F, n, cols, blow, nonzero=nz), 5, 1, torch)
+    ok = bool(torch.equal(back, coeffs)) and int(nz.item()) == 0
+    E.tune_clear()
+    enc_bytes = cols * (n + m) * 4
+    log(f"bench: encoder 256x2^20 BabyBear blowup 4: encode {enc * 1e3:.2f} ms decode {dec * 1e3:.2f} ms "
+        f"round trip ok={ok}; model class {cls} {sched}; held-out regret {res['heldout_mean_regret_model']:.3f}")
+    return {"encode_ms": enc * 1e3, "decode_ms": dec * 1e3, "round_trip_ok": ok,
+            "encode_alg_GBps": enc_bytes / enc / 1e9, "schedule_class": cls, "schedule": sched,
+            "cost_model": {k: v for k, v in res.items() if k != "per_shape"}}
+
+
 def batched(E, F, torch, np):
     """4096 Goldilocks vectors, log2 n ~ U{10..20} (seed 42), mixed forward /
     inverse: graph-replayed persistent plan vs one launch sequence per vector."""
@@ -208,6 +272,11 @@ def main():
         torch.cuda.empty_cache()
         result["sizes_us"] = other_sizes(E, F, torch, np)
         result["batched"] = batched(E, F, torch, np)
+        torch.cuda.empty_cache()
+        result["config1_babybear_2p16"] = config1(E, torch, np)
+        result["config3_partial_2p22"] = config3(E, torch, np)
+        torch.cuda.empty_cache()
+        result["config5_encoder"] = config5(E, torch, np)
     if not args.no_cpu:
         cb, cpu_out = cpu_baseline(host_x, n)
         result["cpu_baseline"] = cb

@@ -4,8 +4,11 @@
 // misc_kernels.hip.  See include/omrntt.h for the contract.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cstdlib>
+#include <string>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -417,8 +420,32 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+extern "C" omr_status omr_tune_load(const char *path);
+
+// The first context loads the shipped tuning table: $OMR_TUNE_FILE, else
+// tuning/mi355x.tune beside the library (written by the exhaustive autotuner,
+// omr_ntt.tune_db).  OMR_TUNE_FILE=none disables it.
+static void load_default_tuning() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const char *env = getenv("OMR_TUNE_FILE");
+    if (env) {
+      if (std::string(env) != "none") (void)omr_tune_load(env);
+      return;
+    }
+    Dl_info info;
+    if (dladdr((void *)&omr_tune_load, &info) && info.dli_fname) {
+      std::string p(info.dli_fname);
+      const size_t slash = p.rfind('/');
+      p = (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/tuning/mi355x.tune";
+      (void)omr_tune_load(p.c_str());
+    }
+  });
+}
+
 extern "C" omr_status omr_ctx_create(int device, omr_ctx **out) {
   if (!out) return OMR_E_NULL;
+  load_default_tuning();
   *out = nullptr;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {

@@ -56,7 +56,7 @@ __device__ __forceinline__ void gen_stage(typename F::T (&x)[16], const typename
       const uint32_t row = ps * EI + ((uint32_t)kk << logs) + jj;
       T v = lds[lpad((row << logC) + c)];
       const uint32_t e = (uint32_t)kk * ps;
-      if (kk != 0 && e != 0) v = F::mulw(v, root[e << root_shift]);
+      if (kk != 0) v = F::mulw(v, root[e << root_shift]);  // e = 0: times root[0] = 1
       x[g * EI + kk] = v;
     }
   }
@@ -127,7 +127,7 @@ k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchV
 #pragma unroll
       for (int m = 0; m < 16; m++) {
         const uint64_t e = (uint64_t)(t + m * RE) * ps;
-        if (e) x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> tb.lo_bits]));
+        x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> tb.lo_bits]));
       }
     }
 

@@ -171,7 +171,7 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
         const uint32_t row = ps * EI + ((uint32_t)kk << LOGS) + jj;
         T v = lds[lpad((row << logC) + c)];
         const uint32_t e = (uint32_t)kk * ps;
-        if (kk != 0 && e != 0) v = F::mulw(v, root[e << root_shift]);
+        if (kk != 0) v = F::mulw(v, root[e << root_shift]);  // e = 0: times root[0] = 1
         x[g * EI + kk] = v;
       }
     }
@@ -288,7 +288,7 @@ k_stockham_pass(PassArgs a) {
 #pragma unroll
         for (int m = 0; m < E; m++) {
           const uint32_t e = (t + m * RE) * (uint32_t)p;
-          if (e) x[m] = F::mulw(x[m], tab[e << sh]);
+          x[m] = F::mulw(x[m], tab[e << sh]);  // e = 0 multiplies by 1: no divergent branch
         }
       } else {  // two-level table
         const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
@@ -297,7 +297,7 @@ k_stockham_pass(PassArgs a) {
 #pragma unroll
         for (int m = 0; m < E; m++) {
           const uint64_t e = (uint64_t)(t + m * RE) * ps;
-          if (e) x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]));
+          x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]));
         }
       }
     }

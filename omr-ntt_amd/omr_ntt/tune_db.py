@@ -1,0 +1,56 @@
+"""Regenerates the engine's shipped tuning table with the exhaustive timing
+autotuner (every schedule class of every shape timed on the GPU).
+
+    python -m omr_ntt.tune_db --out omr-ntt_amd/tuning/mi355x.tune
+
+Full transforms: every field, log2 N = 4 .. min(26, two-adicity), batch 1.
+Partial prefixes: log2 N in {16, 18, 20, 22, 24}, log2 out_len = 1, 3, 5, ...
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import omr_ntt as E  # noqa: E402
+
+
+def build(out, max_log=26, reps=5, partial=True):
+    os.environ["OMR_TUNE_FILE"] = "none"  # start from the untuned defaults
+    E.context()
+    E.tune_clear()
+    report = []
+    t0 = time.time()
+    for F in E.FIELDS:
+        for ln in range(4, min(max_log, F.TWO_ADICITY) + 1):
+            n = 1 << ln
+            c, timed = E.autotune_shape(F, n, n, 1, use_model=False, reps=reps)
+            report.append({"field": F.name, "log_n": ln, "out": ln, "class": c, "timed": timed,
+                           "schedule": E.schedule_class(F, n, n, c)})
+            print(f"{F.name} 2^{ln}: class {c} {report[-1]['schedule']}", flush=True)
+        if partial:
+            for ln in (16, 18, 20, 22, 24):
+                if ln > F.TWO_ADICITY:
+                    continue
+                for lo in range(1, ln, 2):
+                    n = 1 << ln
+                    c, timed = E.autotune_shape(F, n, 1 << lo, 1, use_model=False, reps=reps)
+                    report.append({"field": F.name, "log_n": ln, "out": lo, "class": c, "timed": timed,
+                                   "schedule": E.schedule_class(F, n, 1 << lo, c)})
+            print(f"{F.name}: partial shapes tuned ({time.time() - t0:.0f} s)", flush=True)
+    E.tune_save(out)
+    with open(out + ".json", "w") as fh:
+        json.dump(report, fh, indent=0)
+    print(f"wrote {out} ({len(report)} shapes, {time.time() - t0:.0f} s)")
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--max-log", type=int, default=26)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--no-partial", action="store_true")
+    a = ap.parse_args()
+    build(a.out, a.max_log, a.reps, not a.no_partial)
