@@ -474,6 +474,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
   free_buf(ctx->io[0]);
   free_buf(ctx->io[1]);
   free_buf(ctx->part);
+  free_buf(ctx->part2);
   delete ctx;
   return OMR_OK;
 }

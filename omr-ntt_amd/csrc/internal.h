@@ -60,7 +60,7 @@ struct omr_ctx {
   std::map<std::tuple<int, uint32_t, uint64_t>, omr::TwTables> tables;
   omr::DevBuf scratch[3];
   omr::DevBuf io[2];
-  omr::DevBuf part;
+  omr::DevBuf part, part2;
 };
 
 namespace omr {

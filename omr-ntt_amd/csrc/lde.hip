@@ -16,6 +16,25 @@
 namespace omr {
 static int ilog2v(uint64_t n) { return 63 - __builtin_clzll(n); }
 
+// Runs a column batch in groups whose intermediates (group x 2^logM
+// elements) fit in ~96 MiB, so that the passes after the first read them
+// back from the 256 MB Infinity Cache instead of HBM.
+static omr_status run_columns(omr_ctx *ctx, int field, const void *in, void *out, uint32_t logM,
+                              uint64_t root, const Schedule &s, uint64_t ncols, uint64_t in_stride,
+                              uint64_t out_stride, const XformOpts &o, hipStream_t hs) {
+  const int es = field_bytes(field);
+  const uint64_t per_col = (uint64_t)es << logM;
+  const uint64_t group = std::max<uint64_t>(1, (96ull << 20) / per_col);
+  for (uint64_t c0 = 0; c0 < ncols; c0 += group) {
+    const uint64_t g = std::min(group, ncols - c0);
+    omr_status st = run_transform(ctx, field, (const char *)in + c0 * in_stride * es,
+                                  (char *)out + c0 * out_stride * es, logM, root, s, g, in_stride,
+                                  out_stride, o, hs);
+    if (st != OMR_OK) return st;
+  }
+  return OMR_OK;
+}
+
 static omr_status lde_args(int field, uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t &shift,
                            uint32_t &logM) {
   if (!field_ok(field)) return OMR_E_FIELD;
@@ -57,8 +76,8 @@ extern "C" omr_status omr_coset_lde_device(omr_ctx *ctx, int field, const void *
     o.pre_lo = sh->lo;
     o.pre_hi = sh->hi;
   }
-  return run_transform(ctx, field, d_coeffs, d_evals, logM, w, s, ncols, n, 1ull << logM, o,
-                       (hipStream_t)stream);
+  return run_columns(ctx, field, d_coeffs, d_evals, logM, w, s, ncols, n, 1ull << logM, o,
+                     (hipStream_t)stream);
 }
 
 extern "C" omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const void *d_evals,
@@ -93,5 +112,5 @@ extern "C" omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const voi
   }
   o.store_limit = n;
   o.nonzero_count = d_nonzero;
-  return run_transform(ctx, field, d_evals, d_coeffs, logM, winv, s, ncols, M, n, o, hs);
+  return run_columns(ctx, field, d_evals, d_coeffs, logM, winv, s, ncols, M, n, o, hs);
 }

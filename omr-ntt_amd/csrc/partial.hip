@@ -71,14 +71,30 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
       default_schedule(f, logN, sch);
       return run_transform(ctx, f, d_in, d_out, logN, omega, sch, 1, N, N, o, stream);
     }
-    const uint64_t nt = N / m;
-    const uint32_t nchunks = (uint32_t)std::min<uint64_t>(
-        std::min<uint64_t>(nt, 65535), std::max<uint64_t>(1, (1ull << 20) / m));
-    if ((st = ensure_buf(ctx->part, (size_t)nchunks * m * es)) != OMR_OK) return st;
+    // fold in levels: each level sums ~32 rows of m per thread (x viewed as
+    // rows of m elements) until one row is left in scratch[2]
     if ((st = ensure_buf(ctx->scratch[2], (size_t)m * es)) != OMR_OK) return st;
-    if ((st = launch_fold(f, d_in, ctx->part.p, N, m, nchunks, stream)) != OMR_OK) return st;
-    if ((st = launch_reduce_rows(f, ctx->part.p, ctx->scratch[2].p, nchunks, m, m, stream)) != OMR_OK)
-      return st;
+    const void *cur = d_in;
+    uint64_t rows = N / m;
+    int level = 0;
+    while (rows > 1) {
+      const uint64_t nch = std::min<uint64_t>((rows + 31) / 32, 65535);
+      void *dst;
+      if (nch == 1) {
+        dst = ctx->scratch[2].p;
+      } else {
+        DevBuf &b = (level % 2 == 0) ? ctx->part : ctx->part2;
+        if ((st = ensure_buf(b, (size_t)nch * m * es)) != OMR_OK) return st;
+        dst = b.p;
+      }
+      if ((st = launch_fold(f, cur, dst, rows * m, m, (uint32_t)nch, stream)) != OMR_OK) return st;
+      cur = dst;
+      rows = nch;
+      level++;
+    }
+    if (cur != ctx->scratch[2].p &&
+        hipMemcpyAsync(ctx->scratch[2].p, cur, (size_t)m * es, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+      return OMR_E_DEVICE;
     const uint32_t logm = ilog2u(m);
     if (logm == 0) {
       return hipMemcpyAsync(d_out, ctx->scratch[2].p, es, hipMemcpyDeviceToDevice, stream) == hipSuccess

@@ -4,7 +4,7 @@ autotuner (every schedule class of every shape timed on the GPU).
     python -m omr_ntt.tune_db --out omr-ntt_amd/tuning/mi355x.tune
 
 Full transforms: every field, log2 N = 4 .. min(26, two-adicity), batch 1.
-Partial prefixes: log2 N in {16, 18, 20, 22, 24}, log2 out_len = 1, 3, 5, ...
+Partial prefixes: log2 N in {16, 18, 20, 22, 24}, every log2 out_len < log2 N.
 """
 import argparse
 import json
@@ -34,7 +34,7 @@ def build(out, max_log=26, reps=5, partial=True):
             for ln in (16, 18, 20, 22, 24):
                 if ln > F.TWO_ADICITY:
                     continue
-                for lo in range(1, ln, 2):
+                for lo in range(1, ln):
                     n = 1 << ln
                     c, timed = E.autotune_shape(F, n, 1 << lo, 1, use_model=False, reps=reps)
                     report.append({"field": F.name, "log_n": ln, "out": lo, "class": c, "timed": timed,
