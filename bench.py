@@ -245,6 +245,15 @@ def main():
     alg_bytes = 16 * n
     per_launch = alg_bytes / npass
     achieved = per_launch / (sec / npass) / 1e9
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # (scaled by a 128 MiB copy); used only when it covers this schedule
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_stockham_pass_goldilocks_2p24.json")
+    if args.logn == 24 and os.path.exists(tpath):
+        with open(tpath) as fh:
+            t = json.load(fh)
+        if t.get("schedule_log_radix") == sched:
+            traffic = t["traffic_bytes_per_launch"]
     result = {
         "metric": "Goldilocks 2^24 NTT elems/s & %HBM peak; batched-vs-plain speedup, bit-exact",
         "value": n / sec,
@@ -262,7 +271,8 @@ def main():
                    "schedule_log_radix": sched, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_stockham_pass (one launch per pass)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/traffic_k_stockham_pass_goldilocks_2p24.json" if traffic else None,
                      "alg_bytes_per_launch": per_launch, "launches_per_step": npass},
         "inverse_us": sec_inv * 1e6,
         "bit_exact": {"gpu_round_trip": round_trip},
