@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "omr-ntt_amd"))
 
 HBM_PEAK_GBS = 8000.0
+VALU_LANE_OPS_PER_S = 256 * 64 * 2.4e9  # MI355X: 256 CUs x 4 SIMD16 x 2.4 GHz
 
 
 def log(msg):
@@ -248,12 +249,14 @@ def main():
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # (scaled by a 128 MiB copy); used only when it covers this schedule
     traffic = None
+    valu_ops = None
     tpath = os.path.join(ROOT, "profiles", "traffic_k_stockham_pass_goldilocks_2p24.json")
     if args.logn == 24 and os.path.exists(tpath):
         with open(tpath) as fh:
             t = json.load(fh)
         if t.get("schedule_log_radix") == sched:
             traffic = t["traffic_bytes_per_launch"]
+            valu_ops = sum(x["valu_lane_ops_per_elem"] for x in t["per_pass"])
     result = {
         "metric": "Goldilocks 2^24 NTT elems/s & %HBM peak; batched-vs-plain speedup, bit-exact",
         "value": n / sec,
@@ -275,6 +278,11 @@ def main():
                      "traffic_source": "profiles/traffic_k_stockham_pass_goldilocks_2p24.json" if traffic else None,
                      "alg_bytes_per_launch": per_launch, "launches_per_step": npass},
         "inverse_us": sec_inv * 1e6,
+        # VALU issue limit: measured lane-ops per element (rocprofv3 SQ_INSTS_VALU)
+        # over 256 CUs x 64 lanes x 2.4 GHz
+        "valu": None if valu_ops is None else {
+            "lane_ops_per_elem": valu_ops, "issue_limit_us": valu_ops * n / VALU_LANE_OPS_PER_S * 1e6,
+            "utilization": valu_ops * n / VALU_LANE_OPS_PER_S / sec},
         "bit_exact": {"gpu_round_trip": round_trip},
     }
     if not args.no_extras:
