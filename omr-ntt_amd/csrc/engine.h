@@ -180,37 +180,37 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
   }
 }
 
-// Raw loads of one tile into registers (no arithmetic).  Direct path (s >= C): the thread's E rows of its
-// column.  Staged path (s < C): the thread's E elements tid + m*nthr of the
-// tile's contiguous C*R block, transposed through LDS later.
+// Raw loads of one tile into registers (no arithmetic).  Direct path
+// (s >= C): the thread's E rows of its column.  Staged path (s < C): the
+// thread's E elements tid + m*nthr of the tile's contiguous C*R block,
+// transposed through LDS later.  Offsets inside a vector are 32-bit
+// (N <= 2^32); the zero-tail test runs only when the flag is set.
 template <class F, int LOGR, int LOGE>
-__device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const PassArgs &a,
-                                          uint64_t gt, uint32_t tid, uint32_t c, uint32_t t,
-                                          bool staged) {
+__device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const typename F::T *__restrict__ src,
+                                          const PassArgs &a, uint32_t tile, uint32_t tid, uint32_t c,
+                                          uint32_t t, bool staged) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   const uint32_t logC = a.logC, logS = a.logS;
-  const uint32_t logTiles = a.logN - LOGR - logC;
-  const uint64_t vec = gt >> logTiles, tile = gt & ((1ull << logTiles) - 1);
-  const T *__restrict__ src = (const T *)a.src + vec * a.vec_stride_in;
-  const uint64_t valid = (a.flags & kZeroTail) ? (1ull << a.logValid) : ~0ull;
+  uint32_t pos0, step;
   if (!staged) {
-    const uint64_t q = (tile << logC) + c;
-    const uint64_t S = 1ull << logS;
-    const uint64_t pos0 = (((q >> logS) << LOGR) << logS) + (q & (S - 1));
+    const uint32_t q = (tile << logC) + c;
+    pos0 = (((q >> logS) << LOGR) << logS) + (q & ((1u << logS) - 1)) + (t << logS);
+    step = K::RE << logS;
+  } else {
+    pos0 = (tile << (logC + LOGR)) + tid;
+    step = (1u << logC) * K::RE;
+  }
+  if (a.flags & kZeroTail) {
+    const uint64_t valid = 1ull << a.logValid;
 #pragma unroll
     for (int m = 0; m < K::E; m++) {
-      const uint64_t pos = pos0 + ((uint64_t)(t + m * K::RE) << logS);
-      xn[m] = (pos < valid) ? src[pos] : (T)0;
+      const uint32_t pos = pos0 + (uint32_t)m * step;
+      xn[m] = ((uint64_t)pos < valid) ? src[pos] : (T)0;
     }
   } else {
-    const uint64_t blk0 = tile << (logC + LOGR);
-    const uint32_t nthr = (1u << logC) * K::RE;
 #pragma unroll
-    for (int m = 0; m < K::E; m++) {
-      const uint64_t pos = blk0 + tid + (uint64_t)m * nthr;
-      xn[m] = (pos < valid) ? src[pos] : (T)0;
-    }
+    for (int m = 0; m < K::E; m++) xn[m] = src[pos0 + (uint32_t)m * step];
   }
 }
 
@@ -231,27 +231,27 @@ k_stockham_pass(PassArgs a) {
   const uint32_t tid = threadIdx.x;
   const uint32_t c = tid & (C - 1), t = tid >> logC;
   const uint32_t logS = a.logS;
-  const uint64_t S = 1ull << logS;
+  const uint32_t S = 1u << logS;  // s <= N/R <= 2^31
   const bool staged = S < C;
   const T *__restrict__ root = (const T *)a.root;
   const uint32_t logTiles = a.logN - LOGR - logC;
-  const uint64_t total = (uint64_t)a.nvec << logTiles;
   const uint32_t nthr = C * RE;
 
   T wE[E / 2 + 1];
   load_consts<F, LOGE, U>(wE, root, RE, a.root_shift);
 
   {
-    const uint64_t gt = blockIdx.x;
-    if (gt >= total) return;  // grid = total tiles
-    T x[E];
-    load_tile<F, LOGR, LOGE>(x, a, gt, tid, c, t, staged);
-
-    const uint64_t vec = gt >> logTiles, tile = gt & ((1ull << logTiles) - 1);
+    const uint64_t gt = blockIdx.x;  // grid = nvec << logTiles tiles
+    const uint64_t vec = gt >> logTiles;
+    const uint32_t tile = (uint32_t)(gt & ((1ull << logTiles) - 1));
+    const T *__restrict__ src = (const T *)a.src + vec * a.vec_stride_in;
     T *__restrict__ dst = (T *)a.dst + vec * a.vec_stride_out;
-    const uint64_t q = (tile << logC) + c;
-    const uint64_t p = q >> logS, j = q & (S - 1);
-    const uint64_t pos0 = ((p << LOGR) << logS) + j;
+    T x[E];
+    load_tile<F, LOGR, LOGE>(x, src, a, tile, tid, c, t, staged);
+
+    const uint32_t q = (tile << logC) + c;
+    const uint32_t p = q >> logS, j = q & (S - 1);
+    const uint32_t pos0 = ((p << LOGR) << logS) + j;
 
     if (staged) {
       // element i = (pp R + k) S + jj of the block is column pp S + jj, row k
@@ -259,7 +259,7 @@ k_stockham_pass(PassArgs a) {
 #pragma unroll
       for (int m = 0; m < E; m++) {
         const uint32_t i = tid + (uint32_t)m * nthr;
-        const uint32_t jj = i & (uint32_t)(S - 1);
+        const uint32_t jj = i & (S - 1);
         const uint32_t k = (i >> logS) & (R - 1);
         const uint32_t pp = i >> (logS + LOGR);
         lds[lpad((k << logC) + (pp << logS) + jj)] = x[m];
@@ -271,10 +271,10 @@ k_stockham_pass(PassArgs a) {
 
     if (a.pre_lo) {  // h^pos on input position pos (coset shift)
       const T *plo = (const T *)a.pre_lo, *phi = (const T *)a.pre_hi;
-      const uint64_t lomask = (1ull << a.lo_bits) - 1;
+      const uint32_t lomask = (1u << a.lo_bits) - 1;
 #pragma unroll
       for (int m = 0; m < E; m++) {
-        const uint64_t pos = pos0 + ((uint64_t)(t + m * RE) << logS);
+        const uint32_t pos = pos0 + ((t + m * RE) << logS);
         x[m] = F::mulw(x[m], F::wmul(plo[pos & lomask], phi[pos >> a.lo_bits]));
       }
     }
@@ -287,16 +287,16 @@ k_stockham_pass(PassArgs a) {
         const uint32_t sh = (small ? a.log_rmax : a.log_mid) - logNs;
 #pragma unroll
         for (int m = 0; m < E; m++) {
-          const uint32_t e = (t + m * RE) * (uint32_t)p;
+          const uint32_t e = (t + m * RE) * p;
           x[m] = F::mulw(x[m], tab[e << sh]);  // e = 0 multiplies by 1: no divergent branch
         }
-      } else {  // two-level table
+      } else {  // two-level table (e < N <= 2^32)
         const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
-        const uint64_t ps = p << logS;
-        const uint64_t lomask = (1ull << a.lo_bits) - 1;
+        const uint32_t ps = p << logS;
+        const uint32_t lomask = (1u << a.lo_bits) - 1;
 #pragma unroll
         for (int m = 0; m < E; m++) {
-          const uint64_t e = (uint64_t)(t + m * RE) * ps;
+          const uint32_t e = (t + m * RE) * ps;
           x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]));
         }
       }
@@ -310,6 +310,7 @@ k_stockham_pass(PassArgs a) {
     constexpr int EL = 1 << K::LOGLAST;
     const uint32_t logNR = a.logN - LOGR;
     const bool scale = (a.flags & kScale) != 0;
+    const bool pruned = a.store_limit < (1ull << a.logN);
     const T sw = (T)a.scale_w;
     uint32_t nonzero = 0;
 #pragma unroll
@@ -319,12 +320,12 @@ k_stockham_pass(PassArgs a) {
         const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
         T v = x[g * EL + kk];
         if (scale) v = F::mulw(v, sw);
-        const uint64_t oidx = ((uint64_t)krow << logNR) + q;
+        const uint32_t oidx = (krow << logNR) + q;
         if (a.post_lo) {
           const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
-          v = F::mulw(v, F::wmul(plo[oidx & ((1ull << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
+          v = F::mulw(v, F::wmul(plo[oidx & ((1u << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
         }
-        if (oidx < a.store_limit) dst[oidx] = v;
+        if (!pruned || (uint64_t)oidx < a.store_limit) dst[oidx] = v;
         else nonzero += (v != 0);
       }
     }

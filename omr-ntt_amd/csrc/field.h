@@ -54,10 +54,19 @@ struct Mont32 {
       return a < nb ? r + P : r;
     }
   }
-  OMR_HD static T sub(T a, T b) {
-    uint32_t r = a - b;
-    return a < b ? r + P : r;
+  // For p < 2^31 a difference d in (-p, p) held mod 2^32 is reduced by
+  // min(d, d + p): a wrapped d is >= 2^32 - p > p while d + p < p, and an
+  // unwrapped d < p is below d + p < 2^32.  For p > 2^31 the borrow decides.
+  OMR_HD static T fix_diff(uint32_t d, bool borrow) {
+    if constexpr (!BIG) {
+      (void)borrow;
+      const uint32_t e = d + P;
+      return e < d ? e : d;
+    } else {
+      return borrow ? d + P : d;
+    }
   }
+  OMR_HD static T sub(T a, T b) { return fix_diff(a - b, a < b); }
   OMR_HD static T neg(T a) { return a ? P - a : 0; }
   // Montgomery product a * b * 2^-32 mod p for a < p, b < 2^32.
   OMR_HD static T mont(T a, T b) {
@@ -65,8 +74,7 @@ struct Mont32 {
     uint32_t tlo = (uint32_t)t, thi = (uint32_t)(t >> 32);
     uint32_t m = tlo * PINV;
     uint32_t mh = (uint32_t)(((uint64_t)m * P) >> 32);
-    uint32_t r = thi - mh;
-    return thi < mh ? r + P : r;
+    return fix_diff(thi - mh, thi < mh);
   }
   OMR_HD static T mulw(T a, T w) { return mont(a, w); }
   OMR_HD static T wmul(T w1, T w2) { return mont(w1, w2); }  // W x W -> W
