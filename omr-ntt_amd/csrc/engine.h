@@ -51,15 +51,15 @@ __device__ __forceinline__ constexpr int brev_c(int i) {
   return r;
 }
 
-// Multiply by a register-DFT constant: wE holds multiplier-form roots, or
-// (fields whose small roots are powers of 2) the shift amounts of 2^k.
+// Butterfly difference times a register-DFT constant, (u - v) w_E^idx.
 // U != 0 (fields with power-of-2 roots, N >= 64): w_64 = 2^U, so w_E^idx =
-// 2^((U idx 64/E) mod 192) is a compile-time shift once the DFT loops are
-// unrolled.  U == 0: c is the multiplier-form constant.
+// 2^((U idx 64/E) mod 192) is a compile-time power of 2 once the DFT loops
+// are unrolled, and its sign goes into the order of the subtraction.
+// U == 0: c is the multiplier-form constant.
 template <class F, int U, int LOGE>
-__device__ __forceinline__ typename F::T const_mul(typename F::T d, typename F::T c, int idx) {
-  if constexpr (U != 0) return F::mul_pow2(d, (uint32_t)((U * idx * (64 >> LOGE)) % 192));
-  else return F::mulw(d, c);
+__device__ __forceinline__ typename F::T diff_mul(typename F::T u, typename F::T v, typename F::T c, int idx) {
+  if constexpr (U != 0) return F::diff_pow2(u, v, (uint32_t)((U * idx * (64 >> LOGE)) % 192));
+  else return F::mulw(F::sub(u, v), c);
 }
 
 // Register-DFT constants w_E^k, k <= E/2: multiplier-form values from the
@@ -95,8 +95,8 @@ __device__ __forceinline__ void dft_reg(typename F::T (&x)[1 << LOGE],
       for (int j = 0; j < half; j++) {
         T u = x[OFF + b + j], v = x[OFF + b + j + half];
         x[OFF + b + j] = F::add(u, v);
-        T d = F::sub(u, v);
-        x[OFF + b + j + half] = (j == 0) ? d : const_mul<F, U, LOGE>(d, wE[j * (E / len)], j * (E / len));
+        x[OFF + b + j + half] =
+            (j == 0) ? F::sub(u, v) : diff_mul<F, U, LOGE>(u, v, wE[j * (E / len)], j * (E / len));
       }
     }
   }
@@ -219,8 +219,7 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
 // per-tile invariants live across iterations: 2-3x the VGPRs and a quarter
 // of the occupancy, measured slower.)
 template <class F, int LOGR, int LOGE, int U>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, PassKernel<F, LOGR, LOGE>::MAXT)))
-k_stockham_pass(PassArgs a) {
+__device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   constexpr int R = K::R, E = K::E, RE = K::RE;
@@ -331,6 +330,12 @@ k_stockham_pass(PassArgs a) {
     }
     if (a.nonzero_count && nonzero) atomicAdd(a.nonzero_count, nonzero);
   }
+}
+
+template <class F, int LOGR, int LOGE, int U>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, PassKernel<F, LOGR, LOGE>::MAXT)))
+k_stockham_pass(PassArgs a) {
+  stockham_pass_body<F, LOGR, LOGE, U>(a);
 }
 
 }  // namespace omr

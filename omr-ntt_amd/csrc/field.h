@@ -97,14 +97,19 @@ struct Goldilocks {
   static constexpr int TWO_ADICITY = 32;
   static constexpr int ID = kGoldilocks;
 
+  OMR_HD static T pack(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
   OMR_HD static T add(T a, T b) {  // canonical in and out
     uint64_t s = a + b;
     bool carry = s < a;
     return (carry || s >= P) ? s - P : s;  // on carry s - P == s + EPS mod 2^64
   }
+  // 32-bit borrow chain (v_sub_co / v_subb); a borrow out of the top word
+  // adds P (mod 2^64).
   OMR_HD static T sub(T a, T b) {
-    uint64_t r = a - b;
-    return a < b ? r - EPS : r;  // r + P mod 2^64
+    uint32_t b1, b2;
+    const uint32_t lo = __builtin_subc((uint32_t)a, (uint32_t)b, 0u, &b1);
+    const uint32_t hi = __builtin_subc((uint32_t)(a >> 32), (uint32_t)(b >> 32), b1, &b2);
+    return pack(lo, hi) + (b2 ? P : 0);
   }
   OMR_HD static T neg(T a) { return a ? P - a : 0; }
   // 128-bit product from four 32x32->64 partial products.
@@ -118,14 +123,22 @@ struct Goldilocks {
     lo = (p10 << 32) | (uint32_t)p00;
     hi = p11;
   }
-  // x = lo + hi * 2^64 -> canonical.
+  // x = lo + hi * 2^64 -> canonical, for any lo, hi < 2^64.  With
+  // 2^64 = 2^32 - 1 and 2^96 = -1:  x = lo + hl 2^32 - hl - hh.  Writing
+  // lo_hi + hl = s + c 2^32 (c a carry) and 2^64 = 2^32 - 1 once more,
+  //   x = (H:lo_lo) - D,  H = s + c < 2^32,  D = hl + hh + c < 2^33,
+  // a value in (-2^33, 2^64): a borrow adds P, otherwise one conditional
+  // subtraction of P.  32-bit carry chains (v_add_co / v_addc / v_sub_co /
+  // v_subb) keep the carries in condition registers.
   OMR_HD static T reduce128(uint64_t lo, uint64_t hi) {
-    uint64_t hh = hi >> 32, hl = hi & EPS;
-    uint64_t t0 = lo - hh;
-    if (lo < hh) t0 -= EPS;
-    uint64_t t1 = (hl << 32) - hl;
-    uint64_t r = t0 + t1;
-    if (r < t1) r += EPS;
+    const uint32_t hh = (uint32_t)(hi >> 32), hl = (uint32_t)hi;
+    uint32_t c, c2, b1, b2;
+    const uint32_t s = __builtin_addc((uint32_t)(lo >> 32), hl, 0u, &c);
+    const uint32_t h = s + c;
+    const uint32_t dl = __builtin_addc(hl, hh, c, &c2);
+    const uint32_t rl = __builtin_subc((uint32_t)lo, dl, 0u, &b1);
+    const uint32_t rh = __builtin_subc(h, c2, b1, &b2);
+    const uint64_t r = pack(rl, rh) + (b2 ? P : 0);
     return r >= P ? r - P : r;
   }
   OMR_HD static T mul(T a, T b) {
@@ -145,6 +158,56 @@ struct Goldilocks {
   static constexpr bool kPow2Roots = true;
   // x * 2^k for 0 < k < 64
   OMR_HD static T shl_reduce(T x, uint32_t k) { return reduce128(x << k, x >> (64 - k)); }
+  // x * 2^k for 0 < k < 32: x 2^k = lo + y2 2^64 with y2 < 2^k, and
+  // 2^64 = 2^32 - 1, so one 32x32 multiply-add folds y2 in.
+  // After a carry t < y2 (2^32 - 1) < 2^63, so t + 2^64 = t + (2^32 - 1) is
+  // below P; without one, t >= P subtracts P -- the same addend either way.
+  OMR_HD static T mul_pow2_lo(T x, uint32_t k) {
+    const uint64_t lo = x << k;
+    const uint32_t y2 = (uint32_t)(x >> (64 - k));
+    const uint64_t t = (uint64_t)y2 * 0xFFFFFFFFu + lo;  // carry out <=> t < lo
+    return t + (((t < lo) | (t >= P)) ? EPS : 0);
+  }
+  // -(x * 2^(32 + j)) for 0 <= j < 32, any x < 2^64.  With y = x 2^j =
+  // y0 + y1 2^32 + y2 2^64 (32-bit words) and 2^64 = 2^32 - 1, 2^96 = -1:
+  //   x 2^(32+j) = (y0 + y1) 2^32 - y1 - y2 = H 2^32 - (y1 + y2 + c),
+  // where y0 + y1 = A + c 2^32 and H = A + c < 2^32.  The negation
+  // L - H 2^32 (L = y1 + y2 + c < 2^34) lies in (-P, 2^34): one borrow fix.
+  OMR_HD static T neg_mul_pow2_hi(T x, uint32_t j) {
+    const uint32_t xh = (uint32_t)(x >> 32);
+    const uint32_t y0 = (uint32_t)x << j;
+    const uint32_t y1 = (uint32_t)(x >> (32 - j));
+    const uint32_t y2 = j ? xh >> (32 - j) : 0u;
+    uint32_t c, c2, br;
+    const uint32_t a = __builtin_addc(y0, y1, 0u, &c);
+    const uint32_t h = a + c;
+    const uint32_t l = __builtin_addc(y1, y2, c, &c2);
+    const uint32_t hi = __builtin_subc(c2, h, 0u, &br);
+    return pack(l, hi) + (br ? P : 0);
+  }
+  // x * 2^-m for 0 < m <= 32, any x < 2^64.  With x = q 2^m + r (r < 2^m)
+  // and 2^-m = 2^(32-m) - 2^(64-m) (multiply by 2^m: 2^32 - 2^64 = 1 - p):
+  //   x 2^-m = q + r 2^(32-m) - r 2^(64-m) = w - u,
+  // u = r 2^(64-m) = (x << (64-m)) <= P - 1 has a zero low word, so one
+  // borrow fix gives the canonical value.
+  OMR_HD static T div_pow2(T x, uint32_t m) {
+    const uint32_t uh = (uint32_t)x << (32 - m);
+    const uint64_t w = (x >> m) + uh;
+    uint32_t br;
+    const uint32_t hi = __builtin_subc((uint32_t)(w >> 32), uh, 0u, &br);
+    return pack((uint32_t)w, hi) + (br ? P : 0);
+  }
+  // (a - b) * 2^k for 0 <= k < 192 with k a constant after unrolling: the
+  // sign of 2^k (2^96 = -1) goes into the order of the subtraction, and for
+  // 64 <= k mod 96 < 96, 2^k = -2^-(96 - k mod 96).  Canonical in and out.
+  OMR_HD static T diff_pow2(T a, T b, uint32_t k) {
+    const bool neg = k >= 96;
+    if (neg) k -= 96;
+    if (k == 0) return neg ? sub(b, a) : sub(a, b);
+    if (k < 32) return mul_pow2_lo(neg ? sub(b, a) : sub(a, b), k);
+    if (k < 64) return neg_mul_pow2_hi(neg ? sub(a, b) : sub(b, a), k - 32);
+    return div_pow2(neg ? sub(a, b) : sub(b, a), 96 - k);
+  }
   // x * 2^k mod p for 0 <= k < 192, x canonical
   OMR_HD static T mul_pow2(T x, uint32_t k) {
     const bool negate = k >= 96;

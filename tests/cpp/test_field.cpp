@@ -1,6 +1,7 @@
 // Host build of the engine's field arithmetic (omr-ntt_amd/csrc/field.h)
 // checked against plain 128-bit remainders: add/sub, Montgomery and
-// Goldilocks products, multiplier-form round trips, Goldilocks x * 2^k.
+// Goldilocks products, multiplier-form round trips, Goldilocks x * 2^k,
+// (a - b) * 2^k with the sign folded, x * 2^-m.
 #include <cstdio>
 #include <random>
 
@@ -43,11 +44,27 @@ int main() {
   const uint64_t p = Goldilocks::P;
   for (int it = 0; it < 20000; it++) {
     uint64_t x = it < 3 ? (uint64_t[]){0, 1, p - 1}[it] : rng() % p;
+    uint64_t y = it < 9 ? (uint64_t[]){0, 1, p - 1}[it % 3] : rng() % p;
+    // arbitrary 64-bit words, including the non-canonical range [p, 2^64)
+    uint64_t z = it < 6 ? (uint64_t[]){0, 1, p - 1, p, ~0ull, ~0ull - 1}[it] : (it & 1 ? rng() : p + rng() % (~0ull - p));
     uint64_t pw = 1;
     for (uint32_t k = 0; k < 192; k++) {
       CHECK(Goldilocks::mul_pow2(x, k) == (uint64_t)(((u128)x * pw) % p), "mul_pow2 k=%u\n", k);
+      CHECK(Goldilocks::diff_pow2(x, y, k) == (uint64_t)((((u128)x + p - y) % p * pw) % p),
+            "diff_pow2 k=%u\n", k);
       pw = (uint64_t)(((u128)pw * 2) % p);
     }
+    for (uint32_t m = 1; m <= 32; m++)  // any 64-bit input
+      CHECK((uint64_t)(((u128)Goldilocks::div_pow2(z, m) << m) % p) == (uint64_t)(z % p) &&
+                Goldilocks::div_pow2(z, m) < p,
+            "div_pow2 m=%u\n", m);
+    for (uint32_t j = 0; j < 32; j++) {
+      uint64_t r = Goldilocks::neg_mul_pow2_hi(z, j);
+      uint64_t want = (uint64_t)(((u128)(z % p) * h_pow<Goldilocks>(2, 32 + j)) % p);
+      CHECK(r < p && (r + want) % p == 0, "neg_mul_pow2_hi j=%u\n", j);
+    }
+    for (uint32_t k = 1; k < 32; k++)
+      CHECK(Goldilocks::mul_pow2_lo(x, k) == (uint64_t)(((u128)x << k) % p), "mul_pow2_lo k=%u\n", k);
   }
   if (fails) {
     std::printf("%d failures\n", fails);
