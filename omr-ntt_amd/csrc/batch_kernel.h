@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kTileThreads)
 k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchVec *__restrict__ vecs,
               const BatchTables *__restrict__ tsets, BatchBases bases) {
   using T = typename F::T;
-  __shared__ T lds[kTileElems + kTileElems / 32 + 1];
+  __shared__ T lds[lds_padded(kTileElems)];
   const uint32_t tid = threadIdx.x;
 
   for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {

@@ -47,12 +47,25 @@ unsigned pass_grid(const void *kernel, unsigned block, uint64_t total_tiles);
 hipError_t launch_pass(int field, int logR, int logE, int unit, const PassArgs &a, uint64_t total_tiles,
                        dim3 block, hipStream_t stream);
 
+// The cheapest pass-kernel variant that honours every option set in `a`.
+inline int pass_epilogue(const PassArgs &a) {
+  if (a.pre_lo || a.post_lo || a.nonzero_count || (a.flags & kZeroTail) || a.store_limit < (1ull << a.logN))
+    return kEpiGeneral;
+  return (a.flags & kScale) ? kEpiScale : kEpiPlain;
+}
+
 // Per-field entry points (one translation unit each).
-hipError_t launch_pass_babybear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
-hipError_t launch_pass_koalabear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
-hipError_t launch_pass_goldilocks_u0(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
-hipError_t launch_pass_goldilocks_u39(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
-hipError_t launch_pass_goldilocks_u153(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
-hipError_t launch_pass_teddybear(int logR, int logE, const PassArgs &a, uint64_t t, dim3 b, hipStream_t s);
+hipError_t launch_pass_babybear(int logR, int logE, int epi, const PassArgs &a, uint64_t t, dim3 b,
+                                hipStream_t s);
+hipError_t launch_pass_koalabear(int logR, int logE, int epi, const PassArgs &a, uint64_t t, dim3 b,
+                                 hipStream_t s);
+hipError_t launch_pass_goldilocks_u0(int logR, int logE, int epi, const PassArgs &a, uint64_t t, dim3 b,
+                                     hipStream_t s);
+hipError_t launch_pass_goldilocks_u39(int logR, int logE, int epi, const PassArgs &a, uint64_t t, dim3 b,
+                                      hipStream_t s);
+hipError_t launch_pass_goldilocks_u153(int logR, int logE, int epi, const PassArgs &a, uint64_t t, dim3 b,
+                                       hipStream_t s);
+hipError_t launch_pass_teddybear(int logR, int logE, int epi, const PassArgs &a, uint64_t t, dim3 b,
+                                 hipStream_t s);
 
 }  // namespace omr

@@ -44,6 +44,15 @@ struct PassArgs {
 };
 enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u };
 
+// Pass-kernel variants by the options they honour (chosen per launch): the
+// options' per-element tests, if compiled in, cost VALU and split the store
+// loop into small blocks the compiler cannot interleave.
+//   kEpiPlain   -- kTwiddle only
+//   kEpiScale   -- kTwiddle, every output times scale_w (inverse transforms)
+//   kEpiGeneral -- everything: zero tail, pre/post multipliers, scale,
+//                  store_limit pruning and the nonzero count
+enum : int { kEpiPlain = 0, kEpiScale = 1, kEpiGeneral = 2 };
+
 template <int LOGX>
 __device__ __forceinline__ constexpr int brev_c(int i) {
   int r = 0;
@@ -122,7 +131,10 @@ struct GroupDft<F, LOGE, LOGX, 0, U> {
                                              const typename F::T (&)[(1 << LOGE) / 2 + 1]) {}
 };
 
-constexpr int kLdsBudget = 32768;  // bytes of tile data per workgroup
+#ifndef OMR_LDS_BUDGET
+#define OMR_LDS_BUDGET 32768
+#endif
+constexpr int kLdsBudget = OMR_LDS_BUDGET;  // bytes of tile data per workgroup
 
 template <class F, int LOGR, int LOGE>
 struct PassKernel {
@@ -140,7 +152,11 @@ struct PassKernel {
 
 // Padded LDS index: one spare slot every 32 elements spreads the power-of-two
 // strides of the late in-tile stages over the banks.
-__device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
+#ifndef OMR_LDS_PAD
+#define OMR_LDS_PAD 1
+#endif
+__device__ __forceinline__ uint32_t lpad(uint32_t i) { return OMR_LDS_PAD ? i + (i >> 5) : i; }
+constexpr int lds_padded(int n) { return OMR_LDS_PAD ? n + n / 32 + 1 : n; }
 
 // In-tile stage ST >= 1 of the R-point Stockham: reads the previous stage's
 // outputs from LDS, applies w_R^(k' p s), runs the register DFTs.
@@ -185,7 +201,7 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
 // thread's E elements tid + m*nthr of the tile's contiguous C*R block,
 // transposed through LDS later.  Offsets inside a vector are 32-bit
 // (N <= 2^32); the zero-tail test runs only when the flag is set.
-template <class F, int LOGR, int LOGE>
+template <class F, int LOGR, int LOGE, int EPI>
 __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const typename F::T *__restrict__ src,
                                           const PassArgs &a, uint32_t tile, uint32_t tid, uint32_t c,
                                           uint32_t t, bool staged) {
@@ -201,7 +217,7 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
     pos0 = (tile << (logC + LOGR)) + tid;
     step = (1u << logC) * K::RE;
   }
-  if (a.flags & kZeroTail) {
+  if (EPI == kEpiGeneral && (a.flags & kZeroTail)) {
     const uint64_t valid = 1ull << a.logValid;
 #pragma unroll
     for (int m = 0; m < K::E; m++) {
@@ -218,12 +234,12 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
 // grid-stride loop over several tiles per workgroup made the compiler keep
 // per-tile invariants live across iterations: 2-3x the VGPRs and a quarter
 // of the occupancy, measured slower.)
-template <class F, int LOGR, int LOGE, int U>
+template <class F, int LOGR, int LOGE, int U, int EPI>
 __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   constexpr int R = K::R, E = K::E, RE = K::RE;
-  __shared__ T lds[K::LDS_N + K::LDS_N / 32 + 1];
+  __shared__ T lds[lds_padded(K::LDS_N)];
 
   const uint32_t logC = a.logC;
   const uint32_t C = 1u << logC;
@@ -246,7 +262,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     const T *__restrict__ src = (const T *)a.src + vec * a.vec_stride_in;
     T *__restrict__ dst = (T *)a.dst + vec * a.vec_stride_out;
     T x[E];
-    load_tile<F, LOGR, LOGE>(x, src, a, tile, tid, c, t, staged);
+    load_tile<F, LOGR, LOGE, EPI>(x, src, a, tile, tid, c, t, staged);
 
     const uint32_t q = (tile << logC) + c;
     const uint32_t p = q >> logS, j = q & (S - 1);
@@ -268,7 +284,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
       for (int m = 0; m < E; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
     }
 
-    if (a.pre_lo) {  // h^pos on input position pos (coset shift)
+    if (EPI == kEpiGeneral && a.pre_lo) {  // h^pos on input position pos (coset shift)
       const T *plo = (const T *)a.pre_lo, *phi = (const T *)a.pre_hi;
       const uint32_t lomask = (1u << a.lo_bits) - 1;
 #pragma unroll
@@ -308,8 +324,8 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
     constexpr int EL = 1 << K::LOGLAST;
     const uint32_t logNR = a.logN - LOGR;
-    const bool scale = (a.flags & kScale) != 0;
-    const bool pruned = a.store_limit < (1ull << a.logN);
+    const bool scale = EPI == kEpiScale || (EPI == kEpiGeneral && (a.flags & kScale) != 0);
+    const bool pruned = EPI == kEpiGeneral && a.store_limit < (1ull << a.logN);
     const T sw = (T)a.scale_w;
     uint32_t nonzero = 0;
 #pragma unroll
@@ -320,7 +336,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
         T v = x[g * EL + kk];
         if (scale) v = F::mulw(v, sw);
         const uint32_t oidx = (krow << logNR) + q;
-        if (a.post_lo) {
+        if (EPI == kEpiGeneral && a.post_lo) {
           const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
           v = F::mulw(v, F::wmul(plo[oidx & ((1u << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
         }
@@ -328,14 +344,14 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
         else nonzero += (v != 0);
       }
     }
-    if (a.nonzero_count && nonzero) atomicAdd(a.nonzero_count, nonzero);
+    if (EPI == kEpiGeneral && a.nonzero_count && nonzero) atomicAdd(a.nonzero_count, nonzero);
   }
 }
 
-template <class F, int LOGR, int LOGE, int U>
+template <class F, int LOGR, int LOGE, int U, int EPI>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, PassKernel<F, LOGR, LOGE>::MAXT)))
 k_stockham_pass(PassArgs a) {
-  stockham_pass_body<F, LOGR, LOGE, U>(a);
+  stockham_pass_body<F, LOGR, LOGE, U, EPI>(a);
 }
 
 }  // namespace omr

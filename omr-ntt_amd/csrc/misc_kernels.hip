@@ -126,14 +126,15 @@ omr_status launch_field_op(int f, int op, const void *a, const void *b, void *ou
 
 hipError_t launch_pass(int field, int logR, int logE, int unit, const PassArgs &a, uint64_t total_tiles,
                        dim3 block, hipStream_t stream) {
+  const int epi = pass_epilogue(a);
   switch (field) {
-    case kBabyBear: return launch_pass_babybear(logR, logE, a, total_tiles, block, stream);
-    case kKoalaBear: return launch_pass_koalabear(logR, logE, a, total_tiles, block, stream);
+    case kBabyBear: return launch_pass_babybear(logR, logE, epi, a, total_tiles, block, stream);
+    case kKoalaBear: return launch_pass_koalabear(logR, logE, epi, a, total_tiles, block, stream);
     case kGoldilocks:
-      if (unit == 39) return launch_pass_goldilocks_u39(logR, logE, a, total_tiles, block, stream);
-      if (unit == 153) return launch_pass_goldilocks_u153(logR, logE, a, total_tiles, block, stream);
-      return launch_pass_goldilocks_u0(logR, logE, a, total_tiles, block, stream);
-    case kTeddyBear: return launch_pass_teddybear(logR, logE, a, total_tiles, block, stream);
+      if (unit == 39) return launch_pass_goldilocks_u39(logR, logE, epi, a, total_tiles, block, stream);
+      if (unit == 153) return launch_pass_goldilocks_u153(logR, logE, epi, a, total_tiles, block, stream);
+      return launch_pass_goldilocks_u0(logR, logE, epi, a, total_tiles, block, stream);
+    case kTeddyBear: return launch_pass_teddybear(logR, logE, epi, a, total_tiles, block, stream);
   }
   return hipErrorInvalidValue;
 }

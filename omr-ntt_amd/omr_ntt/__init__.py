@@ -69,6 +69,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with python omr-ntt_amd/build.py")
+        # The PyTorch wheel carries its own HIP runtime (device tensors handed
+        # to the engine come from it).  It must initialise the GPU before this
+        # library's runtime does: in the other order torch finds no device.
+        try:
+            import torch
+            if torch.cuda.device_count() > 0:
+                torch.cuda.init()
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         i32, u64, u32 = ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32
         P64, Pi = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)
