@@ -176,6 +176,110 @@ def batched(E, F, torch, np):
     return res
 
 
+def _pmc_per_dispatch(run_dir, counter, match):
+    """{dispatch_id: summed counter value} for the kernels whose name passes
+    `match`, from one rocprofv3 --pmc run directory."""
+    import glob
+    import sqlite3
+    dbs = glob.glob(os.path.join(run_dir, "**", "*results.db"), recursive=True)
+    if not dbs:
+        raise FileNotFoundError(f"no rocprofv3 results database under {run_dir}")
+    out = {}
+    with sqlite3.connect(dbs[0]) as db:
+        rows = db.execute("select dispatch_id, name, counter_name, counter_value from pmc_events "
+                          "where counter_name = ?", (counter,))
+        for did, name, _, val in rows:
+            if match(name):
+                out[did] = out.get(did, 0.0) + float(val)
+    return out
+
+
+def summarize_profiles(prof_dir, out_path, logn=24):
+    """Per-pass HBM traffic and VALU work of the 2^N Goldilocks forward
+    transform from five rocprofv3 --pmc runs under prof_dir (copy_fetch,
+    copy_write: `bench.py --copy-calibration`; fetch, write, valu: the bench
+    itself), scaled by the copy of known size.  Writes the JSON bench.py
+    reads for roofline.traffic and valu, and returns it."""
+    n = 1 << logn
+    copy_bytes = (1 << 24) * 8  # --copy-calibration: 128 MiB read + 128 MiB written per copy
+    def is_copy(name):
+        return "copy" in name.lower()
+
+    def fwd(name):  # the forward transform's pass kernels: Goldilocks, register-DFT unit 39
+        return "k_stockham_pass<omr::Goldilocks," in name and (", 39," in name or ", 39>" in name)
+    cf = _pmc_per_dispatch(os.path.join(prof_dir, "copy_fetch"), "FETCH_SIZE", is_copy)
+    cw = _pmc_per_dispatch(os.path.join(prof_dir, "copy_write"), "WRITE_SIZE", is_copy)
+    fetch_scale = copy_bytes / 1024 / (sum(cf.values()) / len(cf))
+    write_scale = copy_bytes / 1024 / (sum(cw.values()) / len(cw))
+    f = _pmc_per_dispatch(os.path.join(prof_dir, "fetch"), "FETCH_SIZE", fwd)
+    w = _pmc_per_dispatch(os.path.join(prof_dir, "write"), "WRITE_SIZE", fwd)
+    v = _pmc_per_dispatch(os.path.join(prof_dir, "valu"), "SQ_INSTS_VALU", fwd)
+    sys.path.insert(0, os.path.join(ROOT, "omr-ntt_amd"))
+    import omr_ntt as E
+    sched = E.default_schedule(E.Goldilocks, n)
+    npass = len(sched)
+
+    def per_pass(d):  # skip the first transform (warm-up), average each pass index
+        vals = [d[k] for k in sorted(d)][npass:]
+        vals = vals[:len(vals) // npass * npass]
+        return [sum(vals[i::npass]) / len(vals[i::npass]) for i in range(npass)]
+    fk, wk, vi = per_pass(f), per_pass(w), per_pass(v)
+    passes = [{"pass": i, "fetch_KiB_raw": fk[i], "write_KiB_raw": wk[i],
+               "fetch_bytes": fk[i] * 1024 * fetch_scale, "write_bytes": wk[i] * 1024 * write_scale,
+               "valu_wave_instr": vi[i], "valu_lane_ops_per_elem": vi[i] * 64 / n} for i in range(npass)]
+    res = {"kernel": "k_stockham_pass", "workload": f"Goldilocks forward NTT 2^{logn}",
+           "schedule_log_radix": sched,
+           "traffic_bytes_per_launch": sum(p["fetch_bytes"] + p["write_bytes"] for p in passes) / npass,
+           "fetch_scale": fetch_scale, "write_scale": write_scale, "per_pass": passes}
+    with open(out_path, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print("| pass | FETCH_SIZE (KiB, raw) | HBM read (MiB) | WRITE_SIZE (KiB, raw) | HBM write (MiB) "
+          "| VALU lane-ops / element |")
+    print("|---|---|---|---|---|---|")
+    for p in passes:
+        print(f"| {p['pass']} | {p['fetch_KiB_raw']:,.0f} | {p['fetch_bytes'] / 2**20:.1f} | "
+              f"{p['write_KiB_raw']:,.0f} | {p['write_bytes'] / 2**20:.1f} | {p['valu_lane_ops_per_elem']:.1f} |")
+    print(f"scale factors: FETCH_SIZE x{fetch_scale:.3f}, WRITE_SIZE x{write_scale:.3f}; "
+          f"traffic per launch {res['traffic_bytes_per_launch'] / 1e6:.1f} MB; "
+          f"VALU lane-ops per element {sum(p['valu_lane_ops_per_elem'] for p in passes):.1f}")
+    return res
+
+
+def summarize_trace(trace_dir, logn=24):
+    """Kernel-duration table of a rocprofv3 --kernel-trace run of this bench
+    (calls, mean/min/max microseconds per kernel) and the forward
+    transform's mean per pass in launch order, as markdown."""
+    import glob
+    import sqlite3
+    dbs = glob.glob(os.path.join(trace_dir, "**", "*results.db"), recursive=True)
+    if not dbs:
+        raise FileNotFoundError(f"no rocprofv3 results database under {trace_dir}")
+    with sqlite3.connect(dbs[0]) as db:
+        rows = db.execute("select name, duration, dispatch_id, vgpr_count, lds_size from kernels "
+                          "order by dispatch_id").fetchall()
+    stats = {}
+    for name, dur, _, vg, lds in rows:
+        st = stats.setdefault(name, [0, 0.0, float("inf"), 0.0, vg, lds])
+        st[0] += 1
+        st[1] += dur / 1e3
+        st[2] = min(st[2], dur / 1e3)
+        st[3] = max(st[3], dur / 1e3)
+    print("| kernel | calls | mean (µs) | min | max | VGPRs | LDS (B) |")
+    print("|---|---|---|---|---|---|---|")
+    for name, (k, tot, lo, hi, vg, lds) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{name[:90]}` | {k} | {tot / k:.1f} | {lo:.1f} | {hi:.1f} | {vg} | {lds} |")
+    sys.path.insert(0, os.path.join(ROOT, "omr-ntt_amd"))
+    import omr_ntt as E
+    npass = len(E.default_schedule(E.Goldilocks, 1 << logn))
+    fwd = [d / 1e3 for name, d, _, _, _ in rows
+           if "k_stockham_pass<omr::Goldilocks," in name and (", 39," in name or ", 39>" in name)]
+    fwd = fwd[npass:len(fwd) // npass * npass]  # skip the first transform
+    if fwd:
+        per = [sum(fwd[i::npass]) / len(fwd[i::npass]) for i in range(npass)]
+        print(f"forward transform per pass (launch order): {' / '.join(f'{x:.1f}' for x in per)} µs; "
+              f"sum {sum(per):.1f} µs under the profiler")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,7 +292,19 @@ def main():
                          "rocprof FETCH_SIZE / WRITE_SIZE against a kernel of known traffic")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 2^20/2^22 sizes and the batched uneven-size workload")
+    ap.add_argument("--summarize-profiles", metavar="DIR",
+                    help="no GPU work: summarise the rocprofv3 --pmc runs under DIR (copy_fetch, copy_write, "
+                         "fetch, write, valu) into per-pass HBM traffic and VALU work (see --profile-out)")
+    ap.add_argument("--profile-out", default=os.path.join(ROOT, "profiles", "traffic_k_stockham_pass_goldilocks_2p24.json"))
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="no GPU work: print the kernel-duration table of a rocprofv3 --kernel-trace run under DIR")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        summarize_profiles(args.summarize_profiles, args.profile_out, args.logn)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.logn)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path "
               "(each GPU would run its own share; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
