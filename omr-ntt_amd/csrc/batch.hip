@@ -157,6 +157,9 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
       bt.root = tab->root;
       bt.lo_bits = tab->lo_bits;
       bt.log_rmax = tab->log_rmax;
+      bt.mid = tab->mid;
+      bt.log_mid = tab->log_mid;  // = log_rmax when there is no mid table
+      bt.unit = (uint32_t)tab->unit();
       tset_of[key] = (uint16_t)tsets.size();
       tsets.push_back(bt);
     }

@@ -30,16 +30,17 @@ struct BatchVec {  // one vector in one pass: buffer ids (0 in, 1 out, 2 A, 3 B)
 };
 
 struct BatchTables {  // tables of one root of order 2^logN, multiplier form
-  const void *lo, *hi, *root;
+  const void *lo, *hi, *root, *mid;
   uint64_t scale_w;
-  uint32_t lo_bits, log_rmax;
+  uint32_t lo_bits, log_rmax, log_mid;
+  uint32_t unit;  // register-DFT constants: 0 = table constants, 39 / 153 = 2^k shifts (Goldilocks)
 };
 
 struct BatchBases {
   const void *buf[4];
 };
 
-template <class F, int LEI>
+template <class F, int LEI, int U>
 __device__ __forceinline__ void gen_stage(typename F::T (&x)[16], const typename F::T (&wE)[9],
                                           const typename F::T *lds, const typename F::T *root,
                                           uint32_t t, uint32_t c, uint32_t logC, uint32_t RE,
@@ -60,11 +61,120 @@ __device__ __forceinline__ void gen_stage(typename F::T (&x)[16], const typename
       x[g * EI + kk] = v;
     }
   }
-  GroupDft<F, 4, LEI, 16 / EI>::run(x, wE);
+  GroupDft<F, 4, LEI, 16 / EI, U>::run(x, wE);
 }
 
+// One task (tile) with register-DFT constant unit U (0: table constants).
+template <class F, int U>
+__device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables &tb,
+                                           const BatchVec *__restrict__ vecs, const BatchBases &bases,
+                                           typename F::T *lds, uint32_t tid) {
+  using T = typename F::T;
+  const T *__restrict__ root = (const T *)tb.root;
+  const uint32_t logN = d.logN, logR = d.logR, logS = d.logS;
+  const uint32_t R = 1u << logR, RE = R >> 4;
+  const uint32_t logC = 12 - logR;  // C = 4096 / R columns per task
+  const uint32_t c = tid & ((1u << logC) - 1), t = tid >> logC;
+  const uint32_t logCols = logN - logR;  // columns per vector
+  const uint32_t logCv = logCols < logC ? logCols : logC;
+  const uint32_t u = c >> logCv;  // vector within the task
+  const bool active = u < d.nvec;
+  // offsets inside one vector are 32-bit (N <= 2^32)
+  const uint32_t q = (d.tile << logC) + (c & ((1u << logCv) - 1));
+  const uint32_t S = 1u << logS;
+  const uint32_t p = q >> logS, j = q & (S - 1);
+  const BatchVec bv = vecs[d.vec0 + (active ? u : 0)];
+  const T *__restrict__ src = (const T *)bases.buf[bv.src_buf] + bv.src_off;
+  T *__restrict__ dst = (T *)bases.buf[bv.dst_buf] + bv.dst_off;
+  const uint32_t root_shift = tb.log_rmax - logR;
+
+  T wE[9];
+  load_consts<F, 4, U>(wE, root, RE, root_shift);
+
+  T x[16];
+  if (S >= (1u << logCv)) {
+    const T *base = src + (((p << logR) << logS) + j);
+#pragma unroll
+    for (int m = 0; m < 16; m++) x[m] = active ? base[(t + m * RE) << logS] : (T)0;
+  } else {
+    // each vector of the task contributes the contiguous block
+    // [q0 R, (q0 + Cv) R) of its data, q0 = tile * C
+    const uint32_t vblk = 1u << (logCv + logR);
+    for (uint32_t i = tid; i < (uint32_t)kTileElems; i += kTileThreads) {
+      const uint32_t uu = i >> (logCv + logR);
+      if (uu >= d.nvec) continue;
+      const uint32_t ii = i & (vblk - 1);
+      const uint32_t jj = ii & (S - 1);
+      const uint32_t k = (ii >> logS) & (R - 1);
+      const uint32_t pp = ii >> (logS + logR);
+      const BatchVec v2 = vecs[d.vec0 + uu];
+      const T *s2 = (const T *)bases.buf[v2.src_buf] + v2.src_off;
+      const uint32_t col = (uu << logCv) + (pp << logS) + jj;
+      lds[lpad((k << logC) + col)] = s2[(d.tile << (logC + logR)) + ii];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
+  }
+
+  if (d.flags & kTwiddle) {  // w_N^(k p s) = w_(N/s)^(k p) on input row k
+    const uint32_t logNs = logN - logS;
+    if (logNs <= tb.log_mid) {  // one lookup from the root or the mid table
+      const bool small = logNs <= tb.log_rmax;
+      const T *tab = small ? root : (const T *)tb.mid;
+      const uint32_t sh = (small ? tb.log_rmax : tb.log_mid) - logNs;
+#pragma unroll
+      for (int m = 0; m < 16; m++) x[m] = F::mulw(x[m], tab[((t + m * RE) * p) << sh]);
+    } else {
+      const T *tlo = (const T *)tb.lo, *thi = (const T *)tb.hi;
+      const uint32_t ps = p << logS;
+      const uint32_t lomask = (1u << tb.lo_bits) - 1;
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        const uint32_t e = (t + m * RE) * ps;
+        x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> tb.lo_bits]));
+      }
+    }
+  }
+
+  dft_reg<F, 4, 4, 0, U>(x, wE);
+  const uint32_t nst = (logR + 3) >> 2;
+  const uint32_t loglast = logR - 4 * (nst - 1);
+  for (uint32_t st = 1; st < nst; st++) {
+    const uint32_t logEi = (st == nst - 1) ? loglast : 4;
+    const uint32_t logs = logR - 4 * st - logEi;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++) lds[lpad(((t + m * RE) << logC) + c)] = x[m];
+    __syncthreads();
+    switch (logEi) {
+      case 4: gen_stage<F, 4, U>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+      case 3: gen_stage<F, 3, U>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+      case 2: gen_stage<F, 2, U>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+      default: gen_stage<F, 1, U>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+    }
+  }
+
+  if (d.flags & kScale) {  // a block of its own: the store loop stays branch-free
+    const T sw = (T)tb.scale_w;
+#pragma unroll
+    for (int m = 0; m < 16; m++) x[m] = F::mulw(x[m], sw);
+  }
+  const uint32_t logNR = logN - logR;
+  const uint32_t ELm = (1u << loglast) - 1;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const uint32_t g = (uint32_t)m >> loglast, kk = (uint32_t)m & ELm;
+    const uint32_t krow = kk * (R >> loglast) + t + g * RE;
+    if (active) dst[(krow << logNR) + q] = x[m];
+  }
+}
+
+// 4 waves per SIMD: what the LDS tile allows (4 workgroups of 4 waves per
+// CU).  Goldilocks needs a few dwords of spill for it and still runs 7 %
+// faster than at 3 waves per SIMD without the cap (config-4 batch, measured).
 template <class F>
-__global__ void __launch_bounds__(kTileThreads)
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, kTileThreads), amdgpu_waves_per_eu(4)))
 k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchVec *__restrict__ vecs,
               const BatchTables *__restrict__ tsets, BatchBases bases) {
   using T = typename F::T;
@@ -74,92 +184,12 @@ k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchV
   for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
     const BatchTask d = tasks[ti];
     const BatchTables tb = tsets[d.tset];
-    const T *__restrict__ root = (const T *)tb.root;
-    const uint32_t logN = d.logN, logR = d.logR, logS = d.logS;
-    const uint32_t R = 1u << logR, RE = R >> 4;
-    const uint32_t logC = 12 - logR;  // C = 4096 / R columns per task
-    const uint32_t c = tid & ((1u << logC) - 1), t = tid >> logC;
-    const uint32_t logCols = logN - logR;  // columns per vector
-    const uint32_t logCv = logCols < logC ? logCols : logC;
-    const uint32_t u = c >> logCv;  // vector within the task
-    const bool active = u < d.nvec;
-    const uint64_t q = ((uint64_t)d.tile << logC) + (c & ((1u << logCv) - 1));
-    const uint64_t S = 1ull << logS;
-    const uint64_t p = q >> logS, j = q & (S - 1);
-    const BatchVec bv = vecs[d.vec0 + (active ? u : 0)];
-    const T *__restrict__ src = (const T *)bases.buf[bv.src_buf] + bv.src_off;
-    T *__restrict__ dst = (T *)bases.buf[bv.dst_buf] + bv.dst_off;
-    const uint32_t root_shift = tb.log_rmax - logR;
-
-    T wE[9];
-    load_consts<F, 4>(wE, root, RE, root_shift);
-
-    T x[16];
-    if (S >= (1ull << logCv)) {
-      const T *base = src + ((p << logR) << logS) + j;
-#pragma unroll
-      for (int m = 0; m < 16; m++) x[m] = active ? base[(uint64_t)(t + m * RE) << logS] : (T)0;
+    if constexpr (F::kPow2Roots) {  // per-task register-DFT constants: forward / inverse roots
+      if (tb.unit == 39) batch_task<F, 39>(d, tb, vecs, bases, lds, tid);
+      else if (tb.unit == 153) batch_task<F, 153>(d, tb, vecs, bases, lds, tid);
+      else batch_task<F, 0>(d, tb, vecs, bases, lds, tid);
     } else {
-      // each vector of the task contributes the contiguous block
-      // [q0 R, (q0 + Cv) R) of its data, q0 = tile * C
-      const uint32_t vblk = 1u << (logCv + logR);
-      for (uint32_t i = tid; i < (uint32_t)kTileElems; i += kTileThreads) {
-        const uint32_t uu = i >> (logCv + logR);
-        if (uu >= d.nvec) continue;
-        const uint32_t ii = i & (vblk - 1);
-        const uint32_t jj = ii & (uint32_t)(S - 1);
-        const uint32_t k = (ii >> logS) & (R - 1);
-        const uint32_t pp = ii >> (logS + logR);
-        const BatchVec v2 = vecs[d.vec0 + uu];
-        const T *s2 = (const T *)bases.buf[v2.src_buf] + v2.src_off;
-        const uint32_t col = (uu << logCv) + (pp << logS) + jj;
-        lds[lpad((k << logC) + col)] = s2[((uint64_t)d.tile << (logC + logR)) + ii];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < 16; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
-    }
-
-    if (d.flags & kTwiddle) {
-      const T *tlo = (const T *)tb.lo, *thi = (const T *)tb.hi;
-      const uint64_t ps = p << logS;
-      const uint64_t lomask = (1ull << tb.lo_bits) - 1;
-#pragma unroll
-      for (int m = 0; m < 16; m++) {
-        const uint64_t e = (uint64_t)(t + m * RE) * ps;
-        x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> tb.lo_bits]));
-      }
-    }
-
-    dft_reg<F, 4, 4, 0>(x, wE);
-    const uint32_t nst = (logR + 3) >> 2;
-    const uint32_t loglast = logR - 4 * (nst - 1);
-    for (uint32_t st = 1; st < nst; st++) {
-      const uint32_t logEi = (st == nst - 1) ? loglast : 4;
-      const uint32_t logs = logR - 4 * st - logEi;
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < 16; m++) lds[lpad(((t + m * RE) << logC) + c)] = x[m];
-      __syncthreads();
-      switch (logEi) {
-        case 4: gen_stage<F, 4>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
-        case 3: gen_stage<F, 3>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
-        case 2: gen_stage<F, 2>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
-        default: gen_stage<F, 1>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
-      }
-    }
-
-    const uint32_t logNR = logN - logR;
-    const bool scale = (d.flags & kScale) != 0;
-    const T sw = (T)tb.scale_w;
-    const uint32_t ELm = (1u << loglast) - 1;
-#pragma unroll
-    for (int m = 0; m < 16; m++) {
-      const uint32_t g = (uint32_t)m >> loglast, kk = (uint32_t)m & ELm;
-      const uint32_t krow = kk * (R >> loglast) + t + g * RE;
-      T v = x[m];
-      if (scale) v = F::mulw(v, sw);
-      if (active) dst[((uint64_t)krow << logNR) + q] = v;
+      batch_task<F, 0>(d, tb, vecs, bases, lds, tid);
     }
     __syncthreads();  // LDS is reused by the next task
   }
