@@ -67,6 +67,42 @@ def cpu_baseline(x, n, budget_s=20.0):
                       f"median of {len(times)} runs"}, out
 
 
+def cpu_extras(np, budget_s=4.0):
+    """The other configs' CPU baselines: the C restatements of the reference
+    encoders SURVEY-listed for them, OpenMP over the host cores, median of up
+    to 10 runs within a time budget per item (inputs built outside the
+    timing, like criterion's iter_batched)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    items = [
+        ("config1_babybear_2p16_radix2_dit", O.BABYBEAR, 16, "ArkRadix2", None),
+        ("config2_goldilocks_2p20_winterfell", O.GOLDILOCKS, 20, "WinterfellFourStep", None),
+        ("config2_goldilocks_2p22_winterfell", O.GOLDILOCKS, 22, "WinterfellFourStep", None),
+        ("config2_goldilocks_2p24_stockham8", O.GOLDILOCKS, 24, "TfheStockhamRadix8", None),
+        ("config2_goldilocks_2p24_plonky3", O.GOLDILOCKS, 24, "Plonky3Radix2LayerSplit", None),
+        ("config3_babybear_2p22_td_out2p8", O.BABYBEAR, 22, "TransformDecomposition", 1 << 7),
+        ("config3_goldilocks_2p22_td_out2p8", O.GOLDILOCKS, 22, "TransformDecomposition", 1 << 7),
+        ("config3_babybear_2p22_td_out2p12", O.BABYBEAR, 22, "TransformDecomposition", 1 << 11),
+    ]
+    res = {}
+    for key, f, logn, name, s in items:
+        n = 1 << logn
+        x = O.uniform(f, n, 7)
+        d = O.Domain(f, n)
+        times = []
+        t_start = time.time()
+        while len(times) < 10 and (len(times) < 1 or time.time() - t_start < budget_s):
+            t0 = time.perf_counter()
+            O.encoder(name, f, x, dom=d, s=s)
+            times.append(time.perf_counter() - t0)
+        times.sort()
+        med = times[len(times) // 2]
+        res[key] = {"us": med * 1e6, "elems_per_s": n / med, "encoder": f"{name} (C restatement)",
+                    "runs": len(times), "threads": O.num_threads()}
+        log(f"bench: cpu {key}: {med * 1e3:.2f} ms ({len(times)} runs)")
+    return res
+
+
 def other_sizes(E, F, torch, np):
     out = {}
     for ln in (20, 22):
@@ -414,6 +450,8 @@ def main():
     if not args.no_cpu:
         cb, cpu_out = cpu_baseline(host_x, n)
         result["cpu_baseline"] = cb
+        if not args.no_extras:
+            result["cpu_baselines_other_configs"] = cpu_extras(np)
         x0 = E.to_device(host_x, F)
         y0 = E.empty_device(n, F)
         E.ntt_device(x0, y0, F)
