@@ -136,6 +136,16 @@ omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_in, void *d
                               uint64_t omega, uint64_t out_len, int mode, int method,
                               uint32_t log_n1, void *stream);
 
+/* Prefix outputs of a batch of vectors: out[b][k] = X_b[k] for k < out_len,
+ * vector b at d_in + b*n, its prefix at d_out + b*out_len.  No counterpart
+ * in the reference: the OMR caller's epoch, where every receiver needs
+ * k = 2s outputs of its own N-element vector (benches/common/mod.rs:10-12).
+ * out_len <= 1024: one fused pass per batch (each input read once);
+ * longer prefixes: batched full transforms whose last pass stores only
+ * k < out_len. */
+omr_status omr_partial_batch_device(omr_ctx *ctx, int field, const void *d_in, void *d_out, uint64_t n,
+                                    uint64_t batch, uint64_t omega, uint64_t out_len, void *stream);
+
 /* Coset low-degree extension of `ncols` columns (the encoder path; the
  * reference's decode.rs is a stub, so this is the engine's own closed form):
  *   evals[c][i] = sum_{j<n} coeffs[c][j] (shift * w_m^i)^j,  i < m = n * blowup,
@@ -193,6 +203,77 @@ omr_status omr_batch_set_max_log_radix(int log_radix);
  * multiplier form. */
 omr_status omr_field_op_device(int field, int op, const void *d_a, const void *d_b, void *d_out,
                                uint64_t n, void *stream);
+
+/* ---- autotuner ------------------------------------------------------------
+ * No counterpart in the reference (its schedules are fixed per encoder).  A
+ * shape is (field, n, out_len, batch); candidate schedules come from a fixed
+ * vocabulary of classes: 0..29 full transforms (pass count, pass-radix split,
+ * register radix), 30..35 prefix methods (direct, decomposition with n1 =
+ * 2^6 / 2^8 / 2^10 / largest one-pass radix, full transform with a pruned
+ * last pass).  Tuned choices form a process-wide table consulted by every
+ * transform; the first context loads tuning/mi355x.tune beside the library
+ * ($OMR_TUNE_FILE overrides, "none" disables). */
+int omr_num_schedule_classes(void);
+/* Concrete schedule of class cls for a shape: full classes write up to 8 log
+ * radices, *npass and *log_e; prefix classes write *method and *log_n1.
+ * OMR_E_SCHEDULE if the class is invalid for the shape.  Any output may be
+ * NULL. */
+omr_status omr_schedule_class(int field, uint64_t n, uint64_t out_len, int cls, int *log_radix, int *npass,
+                              int *log_e, int *method, uint32_t *log_n1);
+/* The class the engine uses for a shape without tuning or a model (-1 if the
+ * shape is invalid). */
+int omr_default_class(int field, uint64_t n, uint64_t out_len);
+/* Median microseconds of every valid class over `reps` timed runs on
+ * context-owned pseudo-random inputs: times_us[omr_num_schedule_classes()],
+ * -1 for classes invalid for the shape.  batch > 1 only for full transforms. */
+omr_status omr_tune_classes(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len, uint64_t batch, int reps,
+                            float *times_us);
+/* Installs class cls as the tuned choice for (field, n, out_len). */
+omr_status omr_tune_set_class(int field, uint64_t n, uint64_t out_len, int cls);
+omr_status omr_tune_clear(void);
+/* Text table of the tuned choices (one shape per line). */
+omr_status omr_tune_save(const char *path);
+omr_status omr_tune_load(const char *path);
+
+/* ---- learned cost model -----------------------------------------------------
+ * No counterpart in the reference.  A 4 -> 64 -> 64 -> omr_num_schedule_classes()
+ * fp32 MLP (ReLU) whose outputs are predicted centred log-times of the
+ * classes for a shape's features (field / 3, log n / 32, log out_len / 32,
+ * log batch / 16); trained with Adam on autotuner timings by the engine's
+ * own HIP forward, backward and update kernels.  Parameters are
+ * W1[64][4] b1[64] W2[64][64] b2[64] W3[K][64] b3[K], K = outputs. */
+typedef struct omr_cost_model omr_cost_model;
+omr_status omr_cost_model_create(omr_ctx *ctx, uint32_t seed, omr_cost_model **out);
+omr_status omr_cost_model_destroy(omr_cost_model *cm);
+int omr_cost_model_num_params(void);
+int omr_cost_model_num_inputs(void);
+int omr_cost_model_num_outputs(void);
+omr_status omr_cost_model_get_params(omr_cost_model *cm, float *params);
+omr_status omr_cost_model_set_params(omr_cost_model *cm, const float *params);
+/* Feature vector x[omr_cost_model_num_inputs()] of a shape. */
+omr_status omr_cost_model_features(int field, uint64_t n, uint64_t out_len, uint64_t batch, float *x);
+/* y[B][K] = MLP(x[B][4]) on the GPU. */
+omr_status omr_cost_model_forward(omr_cost_model *cm, const float *x, uint32_t B, float *y);
+/* Loss = mean over (b, k) with mask[b][k] != 0 of (y - t)^2, and its gradient
+ * with respect to every parameter (grad[num_params]), on the GPU. */
+omr_status omr_cost_model_gradients(omr_cost_model *cm, const float *x, const float *t, const float *mask,
+                                    uint32_t B, float *grad, float *loss);
+/* `steps` full-batch Adam steps at learning rate lr; *loss_out = the loss
+ * before the last step. */
+omr_status omr_cost_model_train(omr_cost_model *cm, const float *x, const float *t, const float *mask,
+                                uint32_t B, uint32_t steps, float lr, float *loss_out);
+/* The valid class with the lowest predicted time for a shape. */
+omr_status omr_cost_model_choose(omr_cost_model *cm, int field, uint64_t n, uint64_t out_len, uint64_t batch,
+                                 int *cls);
+omr_status omr_cost_model_save(omr_cost_model *cm, const char *path);
+omr_status omr_cost_model_load(omr_cost_model *cm, const char *path);
+/* Picks and installs a schedule for a shape: use_model = 1 the model's choice
+ * (no timing runs), use_model = k >= 2 the fastest of the model's k
+ * best-predicted classes (k timing runs), use_model = 0 (or an untrained or
+ * NULL model) exhaustive timing of every class.  *cls = the class installed,
+ * *timed = classes timed (may be NULL). */
+omr_status omr_autotune_shape(omr_ctx *ctx, omr_cost_model *cm, int use_model, int field, uint64_t n,
+                              uint64_t out_len, uint64_t batch, int reps, int *cls, int *timed);
 
 #ifdef __cplusplus
 }

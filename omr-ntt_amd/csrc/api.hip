@@ -686,6 +686,19 @@ extern "C" omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_
                       (hipStream_t)stream);
 }
 
+extern "C" omr_status omr_partial_batch_device(omr_ctx *ctx, int field, const void *d_in, void *d_out,
+                                               uint64_t n, uint64_t batch, uint64_t omega, uint64_t out_len,
+                                               void *stream) {
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (out_len == 0 || out_len > n) return OMR_E_OUTLEN;
+  if (batch == 0 || batch > 0xFFFFFFFFull) return OMR_E_ARG;
+  if (!ctx || !d_in || !d_out) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  return partial_batch_impl(ctx, field, d_in, d_out, ilog2(n), batch, omega, out_len, (hipStream_t)stream);
+}
+
 extern "C" omr_status omr_field_op_device(int field, int op, const void *d_a, const void *d_b,
                                           void *d_out, uint64_t n, void *stream) {
   if (!d_a || !d_b || !d_out) return OMR_E_NULL;

@@ -42,7 +42,7 @@ struct PassArgs {
   uint32_t log_rmax;          // root table holds root_(2^log_rmax)^m
   uint32_t log_mid;
 };
-enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u };
+enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u };
 
 // Pass-kernel variants by the options they honour (chosen per launch): the
 // options' per-element tests, if compiled in, cost VALU and split the store
@@ -51,7 +51,12 @@ enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u };
 //   kEpiScale   -- kTwiddle, every output times scale_w (inverse transforms)
 //   kEpiGeneral -- everything: zero tail, pre/post multipliers, scale,
 //                  store_limit pruning and the nonzero count
-enum : int { kEpiPlain = 0, kEpiScale = 1, kEpiGeneral = 2 };
+//   kEpiPrefix  -- (kPrefixSum, the first pass s = N/R of a prefix-partial
+//                  transform) output row k < store_limit of column q times
+//                  w_N^(q k), summed over the tile's columns: one partial sum
+//                  per (k, tile) instead of the pass output
+enum : int { kEpiPlain = 0, kEpiScale = 1, kEpiGeneral = 2, kEpiPrefix = 3 };
+constexpr int kPrefixMaxLogR = 10;  // radices compiled with the prefix epilogue
 
 template <int LOGX>
 __device__ __forceinline__ constexpr int brev_c(int i) {
@@ -323,6 +328,53 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
 
     // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
     constexpr int EL = 1 << K::LOGLAST;
+    if constexpr (EPI == kEpiPrefix) {
+      // With s = N/R, p = 0 and output row k of column q is
+      // Y[q][k] = NTT_R(x[q :: N/R])[k]; the prefix X[k] = sum_q w_N^(q k) Y[q][k]
+      // (the decomposition of transform_decomposition.rs:63-94 with n1 = R).
+      // Per tile (columns q = q0 + c): w_N^(q0 k) sum_c w_N^(c k) Y[q0 + c][k],
+      // the inner exponents < C R from the lo table (one lookup when C R fits).
+      const uint32_t out_len = (uint32_t)a.store_limit;  // <= R
+      const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
+      const uint32_t lomask = (1u << a.lo_bits) - 1;
+      const bool one = (C << LOGR) <= (1u << a.lo_bits);
+      __syncthreads();  // the last in-tile exchange has been read
+#pragma unroll
+      for (int g = 0; g < E / EL; g++) {
+#pragma unroll
+        for (int kk = 0; kk < EL; kk++) {
+          const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
+          if (krow < out_len) {
+            const uint32_t e = krow * c;  // < C R
+            const T w = one ? tlo[e] : F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]);
+            lds[lpad((krow << logC) + c)] = F::mulw(x[g * EL + kk], w);
+          }
+        }
+      }
+      __syncthreads();
+      // partial sums laid out [vector][k][tile]: the reduction over tiles reads
+      // contiguous runs
+      const uint64_t tpv = 1ull << logTiles;
+      T *__restrict__ part = (T *)a.dst + (gt >> logTiles) * tpv * out_len + (gt & (tpv - 1));
+      const uint32_t q0 = tile << logC;
+      const bool big = a.logN > a.log_mid;
+      const T *tab = a.logN <= a.log_rmax ? root : (const T *)a.tw_mid;
+      const uint32_t sh = (a.logN <= a.log_rmax ? a.log_rmax : a.log_mid) - a.logN;
+      for (uint32_t r = tid; r < out_len; r += nthr) {
+        T acc[4] = {0, 0, 0, 0};  // independent chains over the columns
+        uint32_t cc = 0;
+        for (; cc + 4 <= C; cc += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; u++) acc[u] = F::add(acc[u], lds[lpad((r << logC) + cc + u)]);
+        }
+        for (; cc < C; cc++) acc[0] = F::add(acc[0], lds[lpad((r << logC) + cc)]);
+        const T sum = F::add(F::add(acc[0], acc[1]), F::add(acc[2], acc[3]));
+        const uint32_t e = r * q0;  // < N
+        const T w = big ? F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]) : tab[e << sh];
+        part[(uint64_t)r * tpv] = F::mulw(sum, w);
+      }
+      return;
+    }
     const uint32_t logNR = a.logN - LOGR;
     const bool scale = EPI == kEpiScale || (EPI == kEpiGeneral && (a.flags & kScale) != 0);
     const bool pruned = EPI == kEpiGeneral && a.store_limit < (1ull << a.logN);

@@ -102,6 +102,10 @@ omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void
 omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
                         uint64_t omega, uint64_t out_len, int mode, int method, uint32_t log_n1,
                         hipStream_t stream);
+omr_status tune_classes_subset(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len, uint64_t batch, int reps,
+                               const bool *only, float *times_us);
+omr_status partial_batch_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
+                              uint64_t batch, uint64_t omega, uint64_t out_len, hipStream_t stream);
 
 // misc kernel launchers (misc_kernels.hip)
 omr_status launch_scale(int f, const void *in, void *out, uint64_t n, uint64_t w, int apply,
@@ -116,7 +120,10 @@ omr_status launch_td_recombine(int f, const void *Y, void *part, uint64_t out_le
                                uint32_t logN1, uint32_t splits, uint32_t logN, const TwTables *t,
                                hipStream_t s);
 omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m, uint32_t nchunks,
-                       hipStream_t s);
+                       hipStream_t s, uint32_t nvec = 1, uint64_t xs = 0, uint64_t ps = 0);
+// out[v os + k] = sum_{t < n} part[(v m + k) n + t], k < m (<= 2^31), v < nvec (<= 65535)
+omr_status launch_sum_contig(int f, const void *part, void *out, uint64_t n, uint64_t m, uint32_t nvec,
+                             uint64_t os, hipStream_t s);
 omr_status launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 omr_status launch_field_op(int f, int op, const void *a, const void *b, void *out, uint64_t n,
                            hipStream_t s);

@@ -118,21 +118,44 @@ __global__ void __launch_bounds__(256) k_td_recombine(const typename F::T *__res
   if (lane == 0 && k < out_len) part[(uint64_t)blockIdx.y * out_len + k] = red[kl][0];
 }
 
-// Stride-subset fold: part[c * m + i] = sum_{t in chunk c} x[i + m t] (mod p),
-// grid (ceil(m / 256), nchunks); chunk c covers t in [c tpc, (c+1) tpc).
+// Row fold: part[c * m + i] = sum_{t in chunk c} x[i + m t] (mod p), grid
+// (ceil(m / 256), nchunks, nvec); chunk c covers t in [c tpc, (c+1) tpc);
+// set z reads x + z xs and writes part + z ps.
 template <class F>
 __global__ void __launch_bounds__(256) k_fold(const typename F::T *__restrict__ x,
                                               typename F::T *__restrict__ part, uint64_t n,
-                                              uint64_t m, uint64_t tpc) {
+                                              uint64_t m, uint64_t tpc, uint64_t xs, uint64_t ps) {
   using T = typename F::T;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
+  x += (uint64_t)blockIdx.z * xs;
+  part += (uint64_t)blockIdx.z * ps;
   const uint64_t nt = n / m;
   const uint64_t t0 = (uint64_t)blockIdx.y * tpc;
   const uint64_t t1 = t0 + tpc < nt ? t0 + tpc : nt;
   T acc = 0;
   for (uint64_t t = t0; t < t1; t++) acc = F::add(acc, x[i + m * t]);
   part[(uint64_t)blockIdx.y * m + i] = acc;
+}
+
+// Contiguous row sums: out[v os + k] = sum_{t < n} part[(v m + k) n + t],
+// grid (m, nvec), one 256-thread block per sum (coalesced loads, LDS tree).
+template <class F>
+__global__ void __launch_bounds__(256) k_sum_contig(const typename F::T *__restrict__ part,
+                                                    typename F::T *__restrict__ out, uint64_t n, uint64_t m,
+                                                    uint64_t os) {
+  using T = typename F::T;
+  __shared__ T red[256];
+  const T *row = part + ((uint64_t)blockIdx.y * m + blockIdx.x) * n;
+  T acc = 0;
+  for (uint64_t t = threadIdx.x; t < n; t += 256) acc = F::add(acc, row[t]);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = F::add(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(uint64_t)blockIdx.y * os + blockIdx.x] = red[0];
 }
 
 // Elementwise field ops on canonical operands (device arithmetic unit test):

@@ -96,13 +96,23 @@ omr_status launch_td_recombine(int f, const void *Y, void *part, uint64_t out_le
 }
 
 omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m, uint32_t nchunks,
-                       hipStream_t s) {
+                       hipStream_t s, uint32_t nvec, uint64_t xs, uint64_t ps) {
   const uint64_t nt = n / m;
   const uint64_t tpc = (nt + nchunks - 1) / nchunks;
   OMR_FIELD_SWITCH(f, {
     using T = typename F::T;
-    hipLaunchKernelGGL(k_fold<F>, dim3(blocks(m, 256), nchunks), dim3(256), 0, s, (const T *)x,
-                       (T *)part, n, m, tpc);
+    hipLaunchKernelGGL(k_fold<F>, dim3(blocks(m, 256), nchunks, nvec), dim3(256), 0, s, (const T *)x,
+                       (T *)part, n, m, tpc, xs, ps);
+  });
+  return last_launch();
+}
+
+omr_status launch_sum_contig(int f, const void *part, void *out, uint64_t n, uint64_t m, uint32_t nvec,
+                             uint64_t os, hipStream_t s) {
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_sum_contig<F>, dim3((uint32_t)m, nvec), dim3(256), 0, s, (const T *)part, (T *)out, n, m,
+                       os);
   });
   return last_launch();
 }

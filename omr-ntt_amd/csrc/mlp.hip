@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <algorithm>
 #include <vector>
 
 #include "internal.h"
@@ -347,9 +348,11 @@ extern "C" omr_status omr_cost_model_choose(omr_cost_model *cm, int field, uint6
   return OMR_OK;
 }
 
-// Picks and installs a schedule for a shape: from the trained model when
-// use_model != 0 (no timing runs), else -- and whenever the model is
-// untrained -- by exhaustive timing of every valid class.  *timed = number
+// Picks and installs a schedule for a shape.  With a trained model:
+// use_model == 1 takes the model's best-predicted class (no timing runs);
+// use_model = k >= 2 times only the model's k best-predicted valid classes
+// and keeps the fastest (a model-pruned search).  use_model == 0, or an
+// untrained model: exhaustive timing of every valid class.  *timed = number
 // of classes timed.
 extern "C" omr_status omr_autotune_shape(omr_ctx *ctx, omr_cost_model *cm, int use_model, int field,
                                          uint64_t n, uint64_t out_len, uint64_t batch, int reps, int *cls,
@@ -357,8 +360,33 @@ extern "C" omr_status omr_autotune_shape(omr_ctx *ctx, omr_cost_model *cm, int u
   if (!ctx || !cls) return OMR_E_NULL;
   omr_status st;
   int chosen = -1, ntimed = 0;
-  if (use_model && cm && cm->trained) {
+  if (use_model == 1 && cm && cm->trained) {
     if ((st = omr_cost_model_choose(cm, field, n, out_len, batch, &chosen)) != OMR_OK) return st;
+  } else if (use_model >= 2 && cm && cm->trained) {
+    float x[kIn], y[kOut];
+    if ((st = omr_cost_model_features(field, n, out_len, batch, x)) != OMR_OK) return st;
+    if ((st = omr_cost_model_forward(cm, x, 1, y)) != OMR_OK) return st;
+    const uint32_t logN = 63 - __builtin_clzll(n), logOut = 63 - __builtin_clzll(out_len);
+    std::vector<int> valid;
+    for (int c = 0; c < kOut; c++) {
+      Schedule s;
+      int m;
+      uint32_t l1;
+      if (out_len == n ? class_schedule(field, c, logN, s) : class_partial(field, c, logN, logOut, m, l1))
+        valid.push_back(c);
+    }
+    if (valid.empty()) return OMR_E_SCHEDULE;
+    std::stable_sort(valid.begin(), valid.end(), [&](int p, int q) { return y[p] < y[q]; });
+    bool only[kOut] = {};
+    for (size_t i = 0; i < valid.size() && i < (size_t)use_model; i++) only[valid[i]] = true;
+    float t[kOut];
+    if ((st = tune_classes_subset(ctx, field, n, out_len, batch, reps, only, t)) != OMR_OK) return st;
+    for (int c = 0; c < kOut; c++)
+      if (t[c] >= 0.f) {
+        ntimed++;
+        if (chosen < 0 || t[c] < t[chosen]) chosen = c;
+      }
+    if (chosen < 0) return OMR_E_SCHEDULE;
   } else {
     float t[kOut];
     if ((st = omr_tune_classes(ctx, field, n, out_len, batch, reps, t)) != OMR_OK) return st;

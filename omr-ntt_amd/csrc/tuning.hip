@@ -163,6 +163,13 @@ extern "C" omr_status omr_schedule_class(int field, uint64_t n, uint64_t out_len
 // elements in a context-owned buffer.
 extern "C" omr_status omr_tune_classes(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len,
                                        uint64_t batch, int reps, float *times_us) {
+  return tune_classes_subset(ctx, field, n, out_len, batch, reps, nullptr, times_us);
+}
+
+// Times the valid classes with only[cls] set (every valid class when only is
+// null); the others get -1.
+omr_status omr::tune_classes_subset(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len, uint64_t batch,
+                                    int reps, const bool *only, float *times_us) {
   if (!field_ok(field)) return OMR_E_FIELD;
   if (!times_us) return OMR_E_NULL;
   if (n == 0 || (n & (n - 1)) || out_len == 0 || out_len > n || (out_len & (out_len - 1)))
@@ -213,6 +220,7 @@ extern "C" omr_status omr_tune_classes(omr_ctx *ctx, int field, uint64_t n, uint
   (void)hipEventCreate(&e1);
   for (int cls = 0; cls < kNumClasses; cls++) {
     times_us[cls] = -1.f;
+    if (only && !only[cls]) continue;
     Schedule sch;
     int method = 0;
     uint32_t ln1 = 0;

@@ -99,6 +99,7 @@ def lib():
             "omr_default_schedule": (i32, [i32, u64, Pi, Pi]),
             "omr_partial_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, i32, i32, u32, _VP]),
             "omr_field_op_device": (i32, [i32, i32, _VP, _VP, _VP, u64, _VP]),
+            "omr_partial_batch_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, u64, _VP]),
             "omr_batch_plan_create": (i32, [_VP, i32, _VP, u32, ctypes.POINTER(_VP)]),
             "omr_batch_plan_destroy": (i32, [_VP]),
             "omr_batch_plan_info": (i32, [_VP, P64, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
@@ -265,6 +266,17 @@ def partial_device(x, out, field, out_len, mode=PREFIX, method=0, log_n1=0, omeg
     _check_dev_tensor(out, field, out_len)
     _check(lib().omr_partial_device(context(), field.id, _dptr(x), _dptr(out), n, omega, out_len,
                                     mode, method, log_n1, _stream_ptr()), "omr_partial_device")
+    return out
+
+
+def partial_batch_device(x, out, field, n, batch, out_len, omega=None):
+    """Prefix outputs of `batch` vectors of n (the OMR epoch: each receiver's
+    k = 2s outputs): out[b*out_len + k] = NTT(x[b*n:(b+1)*n])[k]."""
+    omega = root_of_unity(field, n) if omega is None else omega
+    _check_dev_tensor(x, field, n * batch)
+    _check_dev_tensor(out, field, out_len * batch)
+    _check(lib().omr_partial_batch_device(context(), field.id, _dptr(x), _dptr(out), n, batch, omega, out_len,
+                                          _stream_ptr()), "omr_partial_batch_device")
     return out
 
 
@@ -459,11 +471,14 @@ class CostModel:
             pass
 
 
-def autotune_shape(field, n, out_len=None, batch=1, model=None, use_model=True, reps=5):
-    """Installs a schedule for the shape: the model's choice (no timing) or
-    exhaustive timing of every class.  Returns (class, classes_timed)."""
+def autotune_shape(field, n, out_len=None, batch=1, model=None, use_model=True, reps=5, top_k=1):
+    """Installs a schedule for the shape: the model's choice (top_k = 1, no
+    timing), the fastest of the model's top_k predicted classes (top_k >= 2,
+    top_k timing runs), or exhaustive timing of every class (use_model
+    False).  Returns (class, classes_timed)."""
     c, timed = ctypes.c_int(), ctypes.c_int()
-    _check(lib().omr_autotune_shape(context(), model._p if model else None, 1 if use_model else 0, field.id, n,
+    mode = max(1, int(top_k)) if use_model else 0
+    _check(lib().omr_autotune_shape(context(), model._p if model else None, mode, field.id, n,
                                     n if out_len is None else out_len, batch, reps, ctypes.byref(c),
                                     ctypes.byref(timed)), "omr_autotune_shape")
     return c.value, timed.value

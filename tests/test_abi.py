@@ -79,6 +79,12 @@ def test_validation_before_device_work():
     assert L.omr_ntt_schedule_device(None, 2, None, None, 1 << 13, 1, E.root_of_unity(gl, 1 << 13), 0, lr, 1, 0, None) == 11
     assert L.omr_partial_device(None, 2, None, None, 1024, w, 2048, 0, 0, 0, None) == 8
     assert L.omr_partial_device(None, 2, None, None, 1024, w, 24, 1, 0, 0, None) == 8
+    # batched prefix: out_len and batch checked before pointers
+    assert L.omr_partial_batch_device(None, 2, None, None, 1024, 4, w, 0, None) == 8
+    assert L.omr_partial_batch_device(None, 2, None, None, 1024, 4, w, 2048, None) == 8
+    assert L.omr_partial_batch_device(None, 2, None, None, 1024, 0, w, 16, None) == 12
+    assert L.omr_partial_batch_device(None, 2, None, None, 1000, 4, w, 16, None) == 2
+    assert L.omr_partial_batch_device(None, 2, None, None, 1024, 4, w, 16, None) == 1
     assert L.omr_variant_name(99) is None
     assert [L.omr_variant_name(i).decode() for i in range(10)] == [
         "ArkRadix2", "Dft", "LambdaBowers", "LambdaRadix4", "Plonky3Radix2LayerSplit",

@@ -229,6 +229,47 @@ def test_partial_large_prefix_and_stride(fname):  # config 3
             assert np.array_equal(E.from_device(out, F), X[:m]), (s, method)
 
 
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_partial_prefix_fused(fname):
+    """Decomposition with the recombine fused into the first pass (out_len <=
+    n1 <= 2^10): every sub-NTT size, odd and power-of-two out_len, against
+    the oracle's full transform; ragged out_len at the smallest sizes."""
+    F = field(fname)
+    for logn in (6, 10, 16, 20):
+        n = 1 << logn
+        x = inputs(F, n, logn)
+        X = O.ntt(F.id, x)
+        t = E.to_device(x, F)
+        for out_len in (1, 2, 5, 64, 100, 256, 1000, 1024):
+            if out_len > n:
+                continue
+            out = E.empty_device(out_len, F)
+            for log_n1 in (0, 6, 7, 8, 10):
+                if log_n1 > logn or (log_n1 and (1 << log_n1) < out_len):
+                    continue
+                E.partial_device(t, out, F, out_len, mode=E.PREFIX, method=2 if log_n1 else 0, log_n1=log_n1)
+                assert np.array_equal(E.from_device(out, F), X[:out_len]), (logn, out_len, log_n1)
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_partial_batch(fname):
+    """The OMR epoch: a batch of receivers, each needing k = 2s outputs of its
+    own vector (benches/common/mod.rs:10-12; s = 1000 is the reference's
+    default), fused (k <= 1024) and full-transform (k > 1024) paths."""
+    F = field(fname)
+    for logn, batch, ks in ((12, 5, (1, 7, 256, 1024, 2000, 4096)), (18, 3, (2000, 200)), (20, 2, (1024,))):
+        n = 1 << logn
+        x = inputs(F, n * batch, logn + batch)
+        t = E.to_device(x, F)
+        want = [O.ntt(F.id, x[b * n:(b + 1) * n]) for b in range(batch)]
+        for k in ks:
+            out = E.empty_device(k * batch, F)
+            E.partial_batch_device(t, out, F, n, batch, k)
+            got = E.from_device(out, F)
+            for b in range(batch):
+                assert np.array_equal(got[b * k:(b + 1) * k], want[b][:k]), (logn, batch, k, b)
+
+
 def test_encoder_registry_names():
     names = [e.name() for e in E.encoders.all(12)]
     assert names == ["ArkRadix2", "LambdaBowers", "WinterfellFourStep", "Plonky3Radix2LayerSplit",

@@ -109,6 +109,10 @@ def test_learned_schedules_regret_and_gain():
     c_model, timed_model = E.autotune_shape(F, n, model=model, use_model=True)
     c_ex, timed_ex = E.autotune_shape(F, n, model=model, use_model=False, reps=2)
     assert timed_model == 0 and timed_ex >= 2
+    # model-pruned search: only the model's 3 best-predicted classes are timed
+    c_top, timed_top = E.autotune_shape(F, n, model=model, use_model=True, top_k=3, reps=2)
+    assert 1 <= timed_top <= 3 < timed_ex
+    assert E.schedule_class(F, n, n, c_top) is not None
     # the installed schedule stays bit-exact
     x = inputs(F, n, 4)
     t = E.to_device(x, F)
