@@ -204,6 +204,24 @@ omr_status omr_batch_set_max_log_radix(int log_radix);
 omr_status omr_field_op_device(int field, int op, const void *d_a, const void *d_b, void *d_out,
                                uint64_t n, void *stream);
 
+/* ---- host-buffer forms --------------------------------------------------------
+ * Synchronous like omr_ntt (copy in, run, copy out) for callers holding host
+ * slices: the SURVEY-listed boundary companions of NttEncoder::ntt
+ * (src/encoder.rs:70-74) -- inverse, batched forward, and the encoder /
+ * decoder the reference's src/decode.rs stubs.  omega is the forward root
+ * the domain holds (domain.omega); same status codes as the device forms. */
+/* buf[0..n) <- inverse NTT (n^-1 sum_k buf[k] omega^-jk), in place. */
+omr_status omr_intt(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t omega);
+/* batch consecutive vectors of n, each forward-transformed in place. */
+omr_status omr_ntt_batched(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t batch, uint64_t omega);
+/* Host forms of omr_coset_lde_device / omr_coset_decode_device (column c of
+ * coeffs at c*n, of evals at c*n*blowup); *nonzero (may be NULL) receives the
+ * decoder's count of nonzero coefficients at index >= n. */
+omr_status omr_coset_lde(omr_ctx *ctx, int field, const void *coeffs, void *evals, uint64_t n, uint64_t ncols,
+                         uint32_t blowup, uint64_t shift);
+omr_status omr_coset_decode(omr_ctx *ctx, int field, const void *evals, void *coeffs, uint64_t n, uint64_t ncols,
+                            uint32_t blowup, uint64_t shift, uint32_t *nonzero);
+
 /* ---- autotuner ------------------------------------------------------------
  * No counterpart in the reference (its schedules are fixed per encoder).  A
  * shape is (field, n, out_len, batch); candidate schedules come from a fixed

@@ -475,6 +475,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
   free_buf(ctx->io[1]);
   free_buf(ctx->part);
   free_buf(ctx->part2);
+  for (auto &b : ctx->hio) free_buf(b);
   delete ctx;
   return OMR_OK;
 }
@@ -741,5 +742,86 @@ extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, u
   }
   if (st != OMR_OK) return st;
   if (hipMemcpy(buf, result, (size_t)copy_back * es, hipMemcpyDeviceToHost) != hipSuccess) return OMR_E_DEVICE;
+  return OMR_OK;
+}
+
+// ---- host-buffer forms of the device entry points ---------------------------
+// Synchronous like omr_ntt: copy in, run on the null stream, copy out.  The
+// device entry point validates first, so a bad argument never touches the GPU.
+
+static omr_status host_stage(omr_ctx *ctx, int i, size_t bytes, const void *src) {
+  omr_status st = ensure_buf(ctx->hio[i], bytes);
+  if (st != OMR_OK) return st;
+  if (src && hipMemcpy(ctx->hio[i].p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_intt(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t omega) {
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (!ctx || !buf) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->host_mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const size_t bytes = (size_t)n * field_bytes(field);
+  if ((st = host_stage(ctx, 0, bytes, buf)) != OMR_OK) return st;
+  if ((st = omr_intt_device(ctx, field, ctx->hio[0].p, ctx->hio[0].p, n, 1, omega, nullptr)) != OMR_OK) return st;
+  return hipMemcpy(buf, ctx->hio[0].p, bytes, hipMemcpyDeviceToHost) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+}
+
+extern "C" omr_status omr_ntt_batched(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t batch,
+                                      uint64_t omega) {
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (batch == 0) return OMR_E_ARG;
+  if (!ctx || !buf) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->host_mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const size_t bytes = (size_t)n * batch * field_bytes(field);
+  if ((st = host_stage(ctx, 0, bytes, buf)) != OMR_OK) return st;
+  if ((st = omr_ntt_device(ctx, field, OMR_ARK_RADIX2, ctx->hio[0].p, ctx->hio[0].p, n, batch, omega, 0,
+                           nullptr)) != OMR_OK)
+    return st;
+  return hipMemcpy(buf, ctx->hio[0].p, bytes, hipMemcpyDeviceToHost) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+}
+
+extern "C" omr_status omr_coset_lde(omr_ctx *ctx, int field, const void *coeffs, void *evals, uint64_t n,
+                                    uint64_t ncols, uint32_t blowup, uint64_t shift) {
+  uint64_t sh = shift;
+  uint32_t logM;
+  omr_status st = lde_args(field, n, ncols, blowup, sh, logM);
+  if (st != OMR_OK) return st;
+  if (!ctx || !coeffs || !evals) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->host_mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const int es = field_bytes(field);
+  const size_t in_bytes = (size_t)n * ncols * es, out_bytes = in_bytes * blowup;
+  if ((st = host_stage(ctx, 0, in_bytes, coeffs)) != OMR_OK) return st;
+  if ((st = host_stage(ctx, 1, out_bytes, nullptr)) != OMR_OK) return st;
+  if ((st = omr_coset_lde_device(ctx, field, ctx->hio[0].p, ctx->hio[1].p, n, ncols, blowup, shift, nullptr)) !=
+      OMR_OK)
+    return st;
+  return hipMemcpy(evals, ctx->hio[1].p, out_bytes, hipMemcpyDeviceToHost) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+}
+
+extern "C" omr_status omr_coset_decode(omr_ctx *ctx, int field, const void *evals, void *coeffs, uint64_t n,
+                                       uint64_t ncols, uint32_t blowup, uint64_t shift, uint32_t *nonzero) {
+  uint64_t sh = shift;
+  uint32_t logM;
+  omr_status st = lde_args(field, n, ncols, blowup, sh, logM);
+  if (st != OMR_OK) return st;
+  if (!ctx || !evals || !coeffs) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->host_mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const int es = field_bytes(field);
+  const size_t out_bytes = (size_t)n * ncols * es, in_bytes = out_bytes * blowup;
+  if ((st = host_stage(ctx, 0, in_bytes, evals)) != OMR_OK) return st;
+  if ((st = host_stage(ctx, 1, out_bytes, nullptr)) != OMR_OK) return st;
+  if (nonzero && (st = host_stage(ctx, 2, sizeof(uint32_t), nullptr)) != OMR_OK) return st;
+  if ((st = omr_coset_decode_device(ctx, field, ctx->hio[0].p, ctx->hio[1].p, n, ncols, blowup, shift,
+                                    nonzero ? (uint32_t *)ctx->hio[2].p : nullptr, nullptr)) != OMR_OK)
+    return st;
+  if (hipMemcpy(coeffs, ctx->hio[1].p, out_bytes, hipMemcpyDeviceToHost) != hipSuccess) return OMR_E_DEVICE;
+  if (nonzero && hipMemcpy(nonzero, ctx->hio[2].p, sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+    return OMR_E_DEVICE;
   return OMR_OK;
 }

@@ -61,6 +61,11 @@ struct omr_ctx {
   omr::DevBuf scratch[3];
   omr::DevBuf io[2];
   omr::DevBuf part, part2;
+  // staging of the host-buffer forms (omr_intt, omr_ntt_batched,
+  // omr_coset_lde, omr_coset_decode), under their own lock: they call the
+  // device entry points, which take `mu`
+  std::mutex host_mu;
+  omr::DevBuf hio[3];
 };
 
 namespace omr {
@@ -102,6 +107,8 @@ omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void
 omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
                         uint64_t omega, uint64_t out_len, int mode, int method, uint32_t log_n1,
                         hipStream_t stream);
+// Argument checks of the coset LDE / decode entry points (shift 0 -> g).
+omr_status lde_args(int field, uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t &shift, uint32_t &logM);
 omr_status tune_classes_subset(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len, uint64_t batch, int reps,
                                const bool *only, float *times_us);
 omr_status partial_batch_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,

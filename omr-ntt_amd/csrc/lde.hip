@@ -35,8 +35,7 @@ static omr_status run_columns(omr_ctx *ctx, int field, const void *in, void *out
   return OMR_OK;
 }
 
-static omr_status lde_args(int field, uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t &shift,
-                           uint32_t &logM) {
+omr_status lde_args(int field, uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t &shift, uint32_t &logM) {
   if (!field_ok(field)) return OMR_E_FIELD;
   if (n == 0 || (n & (n - 1)) || blowup == 0 || (blowup & (blowup - 1))) return OMR_E_SIZE;
   if (ncols == 0 || ncols > 65535) return OMR_E_ARG;

@@ -100,6 +100,10 @@ def lib():
             "omr_partial_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, i32, i32, u32, _VP]),
             "omr_field_op_device": (i32, [i32, i32, _VP, _VP, _VP, u64, _VP]),
             "omr_partial_batch_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, u64, _VP]),
+            "omr_intt": (i32, [_VP, i32, _VP, u64, u64]),
+            "omr_ntt_batched": (i32, [_VP, i32, _VP, u64, u64, u64]),
+            "omr_coset_lde": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64]),
+            "omr_coset_decode": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, ctypes.POINTER(u32)]),
             "omr_batch_plan_create": (i32, [_VP, i32, _VP, u32, ctypes.POINTER(_VP)]),
             "omr_batch_plan_destroy": (i32, [_VP]),
             "omr_batch_plan_info": (i32, [_VP, P64, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
@@ -267,6 +271,57 @@ def partial_device(x, out, field, out_len, mode=PREFIX, method=0, log_n1=0, omeg
     _check(lib().omr_partial_device(context(), field.id, _dptr(x), _dptr(out), n, omega, out_len,
                                     mode, method, log_n1, _stream_ptr()), "omr_partial_device")
     return out
+
+
+def _host_view(buf, field):
+    """(buf, a contiguous array of the field's word type holding the same values)."""
+    if isinstance(buf, np.ndarray) and buf.dtype == field.np_dtype and buf.flags["C_CONTIGUOUS"]:
+        return buf, buf
+    return buf, np.ascontiguousarray(np.asarray(buf).astype(field.np_dtype))
+
+
+def _host_writeback(buf, view, field):
+    if view is not buf:
+        buf[...] = view
+
+
+def intt(buf, field, omega=None):
+    """Host form: inverse NTT of a numpy array in place (omega = the forward root)."""
+    a, view = _host_view(buf, field)
+    n = a.size
+    omega = root_of_unity(field, n) if omega is None else omega
+    _check(lib().omr_intt(context(), field.id, view.ctypes.data, n, omega), "omr_intt")
+    _host_writeback(buf, view, field)
+    return buf
+
+
+def ntt_batched(buf, field, batch, omega=None):
+    """Host form: `batch` consecutive vectors forward-transformed in place."""
+    a, view = _host_view(buf, field)
+    n = a.size // batch
+    omega = root_of_unity(field, n) if omega is None else omega
+    _check(lib().omr_ntt_batched(context(), field.id, view.ctypes.data, n, batch, omega), "omr_ntt_batched")
+    _host_writeback(buf, view, field)
+    return buf
+
+
+def coset_lde(coeffs, field, n, ncols=1, blowup=4, shift=0):
+    """Host form of coset_lde_device: returns the evaluations (numpy)."""
+    _, cv = _host_view(coeffs, field)
+    ev = np.empty(n * ncols * blowup, dtype=field.np_dtype)
+    _check(lib().omr_coset_lde(context(), field.id, cv.ctypes.data, ev.ctypes.data, n, ncols, blowup, shift),
+           "omr_coset_lde")
+    return ev.astype(np.uint64)
+
+
+def coset_decode(evals, field, n, ncols=1, blowup=4, shift=0):
+    """Host form of coset_decode_device: returns (coefficients, nonzero tail count)."""
+    _, ev = _host_view(evals, field)
+    co = np.empty(n * ncols, dtype=field.np_dtype)
+    nz = ctypes.c_uint32(0)
+    _check(lib().omr_coset_decode(context(), field.id, ev.ctypes.data, co.ctypes.data, n, ncols, blowup, shift,
+                                  ctypes.byref(nz)), "omr_coset_decode")
+    return co.astype(np.uint64), nz.value
 
 
 def partial_batch_device(x, out, field, n, batch, out_len, omega=None):

@@ -85,6 +85,16 @@ def test_validation_before_device_work():
     assert L.omr_partial_batch_device(None, 2, None, None, 1024, 0, w, 16, None) == 12
     assert L.omr_partial_batch_device(None, 2, None, None, 1000, 4, w, 16, None) == 2
     assert L.omr_partial_batch_device(None, 2, None, None, 1024, 4, w, 16, None) == 1
+    # host forms: validated like their device forms, before any pointer use
+    assert L.omr_intt(None, 2, None, 1000, w) == 2
+    assert L.omr_intt(None, 2, None, 1024, w) == 1
+    assert L.omr_ntt_batched(None, 2, None, 1024, 0, w) == 12
+    assert L.omr_ntt_batched(None, 2, None, 1024, 3, w) == 1
+    assert L.omr_coset_lde(None, 0, None, None, 1 << 26, 1, 4, 0) == 3   # 2^28 > BabyBear 2-adicity
+    assert L.omr_coset_lde(None, 9, None, None, 1024, 1, 4, 0) == 4
+    assert L.omr_coset_lde(None, 2, None, None, 1024, 1, 3, 0) == 2
+    assert L.omr_coset_lde(None, 2, None, None, 1024, 1, 4, 0) == 1
+    assert L.omr_coset_decode(None, 2, None, None, 1024, 1, 4, 0, None) == 1
     assert L.omr_variant_name(99) is None
     assert [L.omr_variant_name(i).decode() for i in range(10)] == [
         "ArkRadix2", "Dft", "LambdaBowers", "LambdaRadix4", "Plonky3Radix2LayerSplit",

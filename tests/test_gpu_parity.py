@@ -270,6 +270,35 @@ def test_partial_batch(fname):
                 assert np.array_equal(got[b * k:(b + 1) * k], want[b][:k]), (logn, batch, k, b)
 
 
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_host_forms(fname):
+    """Host-buffer forms (copy in, run, copy out): inverse, batched forward,
+    coset LDE and decode, against the oracle."""
+    F = field(fname)
+    n = 1 << 12
+    x = inputs(F, n, 3)
+    X = O.ntt(F.id, x)
+    y = X.copy()
+    E.intt(y, F)
+    assert np.array_equal(y, x)
+    b = 3
+    xs = inputs(F, n * b, 4)
+    ys = xs.copy()
+    E.ntt_batched(ys, F, b)
+    for i in range(b):
+        assert np.array_equal(ys[i * n:(i + 1) * n], O.ntt(F.id, xs[i * n:(i + 1) * n]))
+    m, cols = 256, 2
+    c = inputs(F, m * cols, 5)
+    ev = E.coset_lde(c, F, m, cols, 4)
+    for j in range(cols):
+        assert np.array_equal(ev[j * 4 * m:(j + 1) * 4 * m], O.coset_lde(F.id, c[j * m:(j + 1) * m], 4))
+    back, nz = E.coset_decode(ev, F, m, cols, 4)
+    assert np.array_equal(back, c) and nz == 0
+    ev[7] = (int(ev[7]) + 1) % F.p  # a corrupted codeword has a nonzero tail
+    _, nz = E.coset_decode(ev, F, m, cols, 4)
+    assert nz > 0
+
+
 def test_encoder_registry_names():
     names = [e.name() for e in E.encoders.all(12)]
     assert names == ["ArkRadix2", "LambdaBowers", "WinterfellFourStep", "Plonky3Radix2LayerSplit",
