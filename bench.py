@@ -69,7 +69,7 @@ def cpu_baseline(x, n, budget_s=20.0):
 
 def cpu_extras(np, budget_s=4.0):
     """The other configs' CPU baselines: the C restatements of the reference
-    encoders SURVEY-listed for them, OpenMP over the host cores, median of up
+    encoders that run them, OpenMP over the host cores, median of up
     to 10 runs within a time budget per item (inputs built outside the
     timing, like criterion's iter_batched)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -116,6 +116,33 @@ def other_sizes(E, F, torch, np):
         out[f"2^{ln}"] = {"forward_us": fwd * 1e6, "inverse_us": inv * 1e6, "schedule": E.default_schedule(F, n),
                           "elems_per_s": n / fwd}
         log(f"bench: 2^{ln} forward {fwd * 1e6:.1f} us inverse {inv * 1e6:.1f} us")
+    return out
+
+
+def host_path(E, F, n, host_x, np):
+    """The trait drop-in path (NttEncoder::ntt on a host slice, omr_ntt):
+    copy in over PCIe, transform, copy out.  Timed for a pageable buffer
+    (staged through the library's pinned ring, and with the ring off) and
+    for a buffer from omr_host_alloc (direct DMA).  PCIe-inclusive: never the headline value."""
+    dom = E.NttDomain(n, F)
+    enc = E.ArkRadix2()
+    out = {}
+    for kind in ("pageable", "pageable_unstaged", "pinned"):
+        buf = E.host_empty(n, F) if kind == "pinned" else np.empty(n, dtype=F.np_dtype)
+        times = []
+        if kind == "pageable_unstaged":
+            os.environ["OMR_HOST_STAGING"] = "0"  # plain hipMemcpy of the pageable buffer
+        for _ in range(6):
+            buf[:] = host_x
+            t0 = time.perf_counter()
+            enc.ntt(buf, dom)
+            times.append(time.perf_counter() - t0)
+        os.environ.pop("OMR_HOST_STAGING", None)
+        med = float(np.median(times[1:]))
+        out[kind] = {"ms": med * 1e3, "elems_per_s": n / med,
+                     "pcie_gb_per_s": 2 * n * F.bytes / med / 1e9}
+        log(f"bench: host path ({kind}) {med * 1e3:.2f} ms per call")
+        del buf
     return out
 
 
@@ -347,7 +374,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path "
-              "(each GPU would run its own share; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "(each GPU would run its own share of the work). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -444,6 +471,7 @@ def main():
     if not args.no_extras:
         del xs, ys, back
         torch.cuda.empty_cache()
+        result["host_path"] = host_path(E, F, n, host_x, np)
         result["sizes_us"] = other_sizes(E, F, torch, np)
         result["batched"] = batched(E, F, torch, np)
         torch.cuda.empty_cache()

@@ -206,7 +206,7 @@ omr_status omr_field_op_device(int field, int op, const void *d_a, const void *d
 
 /* ---- host-buffer forms --------------------------------------------------------
  * Synchronous like omr_ntt (copy in, run, copy out) for callers holding host
- * slices: the SURVEY-listed boundary companions of NttEncoder::ntt
+ * slices: the boundary companions of NttEncoder::ntt
  * (src/encoder.rs:70-74) -- inverse, batched forward, and the encoder /
  * decoder the reference's src/decode.rs stubs.  omega is the forward root
  * the domain holds (domain.omega); same status codes as the device forms. */
@@ -221,6 +221,18 @@ omr_status omr_coset_lde(omr_ctx *ctx, int field, const void *coeffs, void *eval
                          uint32_t blowup, uint64_t shift);
 omr_status omr_coset_decode(omr_ctx *ctx, int field, const void *evals, void *coeffs, uint64_t n, uint64_t ncols,
                             uint32_t blowup, uint64_t shift, uint32_t *nonzero);
+
+/* Pinned host memory.  No counterpart in the reference: `NttEncoder::ntt`
+ * takes any `&mut [F]` (src/encoder.rs:70-74).  The host-buffer entry points
+ * copy a pageable buffer of 8 MiB or more through a pinned staging ring
+ * (host threads fill one 16 MiB slot while the DMA engine moves the one
+ * before it; OMR_HOST_THREADS sets the thread count, default min(8, cores);
+ * OMR_HOST_STAGING=0 turns the ring off).
+ * A buffer inside a block from omr_host_alloc is copied directly.  Blocks
+ * stay valid until omr_host_free; omr_host_free of any other pointer returns
+ * OMR_E_ARG. */
+omr_status omr_host_alloc(size_t bytes, void **out);
+omr_status omr_host_free(void *p);
 
 /* ---- autotuner ------------------------------------------------------------
  * No counterpart in the reference (its schedules are fixed per encoder).  A

@@ -476,6 +476,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
   free_buf(ctx->part);
   free_buf(ctx->part2);
   for (auto &b : ctx->hio) free_buf(b);
+  free_stage(ctx->stage);
   delete ctx;
   return OMR_OK;
 }
@@ -721,7 +722,7 @@ extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, u
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
   if ((st = ensure_buf(ctx->io[0], (size_t)n * es)) != OMR_OK) return st;
-  if (hipMemcpy(ctx->io[0].p, buf, (size_t)n * es, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
+  if ((st = host_to_device(ctx, ctx->io[0].p, buf, (size_t)n * es)) != OMR_OK) return st;
   uint64_t copy_back = n;
   void *result = ctx->io[0].p;
   if (!is_partial_variant(variant)) {
@@ -741,8 +742,7 @@ extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, u
     result = ctx->io[1].p;
   }
   if (st != OMR_OK) return st;
-  if (hipMemcpy(buf, result, (size_t)copy_back * es, hipMemcpyDeviceToHost) != hipSuccess) return OMR_E_DEVICE;
-  return OMR_OK;
+  return device_to_host(ctx, buf, result, (size_t)copy_back * es);
 }
 
 // ---- host-buffer forms of the device entry points ---------------------------
@@ -752,8 +752,7 @@ extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, u
 static omr_status host_stage(omr_ctx *ctx, int i, size_t bytes, const void *src) {
   omr_status st = ensure_buf(ctx->hio[i], bytes);
   if (st != OMR_OK) return st;
-  if (src && hipMemcpy(ctx->hio[i].p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
-  return OMR_OK;
+  return src ? host_to_device(ctx, ctx->hio[i].p, src, bytes) : OMR_OK;
 }
 
 extern "C" omr_status omr_intt(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t omega) {
@@ -765,7 +764,7 @@ extern "C" omr_status omr_intt(omr_ctx *ctx, int field, void *buf, uint64_t n, u
   const size_t bytes = (size_t)n * field_bytes(field);
   if ((st = host_stage(ctx, 0, bytes, buf)) != OMR_OK) return st;
   if ((st = omr_intt_device(ctx, field, ctx->hio[0].p, ctx->hio[0].p, n, 1, omega, nullptr)) != OMR_OK) return st;
-  return hipMemcpy(buf, ctx->hio[0].p, bytes, hipMemcpyDeviceToHost) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+  return device_to_host(ctx, buf, ctx->hio[0].p, bytes);
 }
 
 extern "C" omr_status omr_ntt_batched(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t batch,
@@ -781,7 +780,7 @@ extern "C" omr_status omr_ntt_batched(omr_ctx *ctx, int field, void *buf, uint64
   if ((st = omr_ntt_device(ctx, field, OMR_ARK_RADIX2, ctx->hio[0].p, ctx->hio[0].p, n, batch, omega, 0,
                            nullptr)) != OMR_OK)
     return st;
-  return hipMemcpy(buf, ctx->hio[0].p, bytes, hipMemcpyDeviceToHost) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+  return device_to_host(ctx, buf, ctx->hio[0].p, bytes);
 }
 
 extern "C" omr_status omr_coset_lde(omr_ctx *ctx, int field, const void *coeffs, void *evals, uint64_t n,
@@ -800,7 +799,7 @@ extern "C" omr_status omr_coset_lde(omr_ctx *ctx, int field, const void *coeffs,
   if ((st = omr_coset_lde_device(ctx, field, ctx->hio[0].p, ctx->hio[1].p, n, ncols, blowup, shift, nullptr)) !=
       OMR_OK)
     return st;
-  return hipMemcpy(evals, ctx->hio[1].p, out_bytes, hipMemcpyDeviceToHost) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+  return device_to_host(ctx, evals, ctx->hio[1].p, out_bytes);
 }
 
 extern "C" omr_status omr_coset_decode(omr_ctx *ctx, int field, const void *evals, void *coeffs, uint64_t n,
@@ -820,8 +819,6 @@ extern "C" omr_status omr_coset_decode(omr_ctx *ctx, int field, const void *eval
   if ((st = omr_coset_decode_device(ctx, field, ctx->hio[0].p, ctx->hio[1].p, n, ncols, blowup, shift,
                                     nonzero ? (uint32_t *)ctx->hio[2].p : nullptr, nullptr)) != OMR_OK)
     return st;
-  if (hipMemcpy(coeffs, ctx->hio[1].p, out_bytes, hipMemcpyDeviceToHost) != hipSuccess) return OMR_E_DEVICE;
-  if (nonzero && hipMemcpy(nonzero, ctx->hio[2].p, sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
-    return OMR_E_DEVICE;
-  return OMR_OK;
+  if ((st = device_to_host(ctx, coeffs, ctx->hio[1].p, out_bytes)) != OMR_OK) return st;
+  return nonzero ? device_to_host(ctx, nonzero, ctx->hio[2].p, sizeof(uint32_t)) : OMR_OK;
 }

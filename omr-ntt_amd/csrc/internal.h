@@ -52,6 +52,17 @@ struct XformOpts {
   uint32_t *nonzero_count = nullptr;  // counts nonzero outputs at index >= store_limit
 };
 
+// Pinned staging ring of the host-buffer entry points (hostio.hip): kSlots
+// chunks of kChunk bytes, one copy stream and one event per slot.
+struct HostStage {
+  static constexpr size_t kChunk = 16ull << 20;
+  static constexpr int kSlots = 4;
+  std::mutex mu;
+  void *ring = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[kSlots] = {};
+};
+
 }  // namespace omr
 
 struct omr_ctx {
@@ -66,6 +77,7 @@ struct omr_ctx {
   // device entry points, which take `mu`
   std::mutex host_mu;
   omr::DevBuf hio[3];
+  omr::HostStage stage;  // its own lock, taken last
 };
 
 namespace omr {
@@ -108,6 +120,11 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
                         uint64_t omega, uint64_t out_len, int mode, int method, uint32_t log_n1,
                         hipStream_t stream);
 // Argument checks of the coset LDE / decode entry points (shift 0 -> g).
+// Host <-> device copies of the host-buffer entry points: pinned staging for
+// large pageable buffers.  device_to_host first waits for the null stream.
+omr_status host_to_device(omr_ctx *ctx, void *dev, const void *host, size_t bytes);
+omr_status device_to_host(omr_ctx *ctx, void *host, const void *dev, size_t bytes);
+void free_stage(HostStage &s);
 omr_status lde_args(int field, uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t &shift, uint32_t &logM);
 omr_status tune_classes_subset(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len, uint64_t batch, int reps,
                                const bool *only, float *times_us);

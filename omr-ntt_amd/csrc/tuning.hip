@@ -14,7 +14,7 @@
 //     2^10; 34: decomposition with the largest one-pass radix (2^12, or
 //     2^13 for 32-bit fields, at most N); 35: full transform, pruned last pass.
 // The timing autotuner measures every valid class of a shape; its best is
-// the reference the cost model's regret is judged against.
+// the reference the cost model's regret is measured against.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

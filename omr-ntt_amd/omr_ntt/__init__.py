@@ -16,6 +16,7 @@ stream).  There is no CPU fallback: without libomrntt.so every call raises.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -136,6 +137,8 @@ def lib():
             "omr_autotune_shape": (i32, [_VP, _VP, i32, i32, u64, u64, u64, i32, Pi, Pi]),
             "omr_coset_lde_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, _VP]),
             "omr_coset_decode_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u32, u64, _VP, _VP]),
+            "omr_host_alloc": (i32, [ctypes.c_size_t, ctypes.POINTER(_VP)]),
+            "omr_host_free": (i32, [_VP]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -271,6 +274,18 @@ def partial_device(x, out, field, out_len, mode=PREFIX, method=0, log_n1=0, omeg
     _check(lib().omr_partial_device(context(), field.id, _dptr(x), _dptr(out), n, omega, out_len,
                                     mode, method, log_n1, _stream_ptr()), "omr_partial_device")
     return out
+
+
+def host_empty(n, field):
+    """A numpy array of n field words in pinned host memory (omr_host_alloc):
+    the host-buffer entry points copy it directly instead of through their
+    staging ring.  The block is freed when the last view of it goes away."""
+    nbytes = int(n) * field.bytes
+    p = _VP()
+    _check(lib().omr_host_alloc(max(nbytes, 1), ctypes.byref(p)), "omr_host_alloc")
+    block = (ctypes.c_char * max(nbytes, 1)).from_address(p.value)
+    weakref.finalize(block, lib().omr_host_free, p.value)
+    return np.frombuffer(block, dtype=field.np_dtype, count=int(n))
 
 
 def _host_view(buf, field):

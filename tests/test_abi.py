@@ -95,6 +95,13 @@ def test_validation_before_device_work():
     assert L.omr_coset_lde(None, 2, None, None, 1024, 1, 3, 0) == 2
     assert L.omr_coset_lde(None, 2, None, None, 1024, 1, 4, 0) == 1
     assert L.omr_coset_decode(None, 2, None, None, 1024, 1, 4, 0, None) == 1
+    # pinned host blocks: argument checks, and only blocks of omr_host_alloc are freed
+    p = ctypes.c_void_p()
+    assert L.omr_host_alloc(0, ctypes.byref(p)) == 12
+    assert L.omr_host_alloc(64, None) == 1
+    assert L.omr_host_free(None) == 1
+    stray = np.zeros(8, dtype=np.uint64)
+    assert L.omr_host_free(stray.ctypes.data) == 12
     assert L.omr_variant_name(99) is None
     assert [L.omr_variant_name(i).decode() for i in range(10)] == [
         "ArkRadix2", "Dft", "LambdaBowers", "LambdaRadix4", "Plonky3Radix2LayerSplit",

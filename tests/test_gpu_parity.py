@@ -283,7 +283,7 @@ def test_host_forms(fname):
     assert np.array_equal(y, x)
     b = 3
     xs = inputs(F, n * b, 4)
-    ys = xs.copy()
+    ys = xs.astype(F.np_dtype)
     E.ntt_batched(ys, F, b)
     for i in range(b):
         assert np.array_equal(ys[i * n:(i + 1) * n], O.ntt(F.id, xs[i * n:(i + 1) * n]))
@@ -297,6 +297,71 @@ def test_host_forms(fname):
     ev[7] = (int(ev[7]) + 1) % F.p  # a corrupted codeword has a nonzero tail
     _, nz = E.coset_decode(ev, F, m, cols, 4)
     assert nz > 0
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_montgomery_words_pass_through(fname):
+    """Every operation is F-linear, so words in Montgomery form (x R mod p,
+    R = 2^64 for ark-ff's one-limb BigInt, 2^32 for a u32 form) go in and
+    come out in that form: a caller holding Montgomery limbs needs no
+    conversion of the buffer, only of the scalar omega."""
+    F = field(fname)
+    n = 1 << 12
+    x = inputs(F, n, 21)
+    for R in (1 << 64, 1 << 32):
+        r = np.full(n, R % F.p, dtype=np.uint64)
+        mont = lambda v: O.vec_op(F.id, 2, v, r[:v.size])  # noqa: E731
+        assert np.array_equal(gpu_ntt(F, mont(x)), mont(O.ntt(F.id, x)))
+        t = E.to_device(mont(x), F)
+        E.intt_device(t, t, F)
+        assert np.array_equal(E.from_device(t, F), mont(O.intt(F.id, x)))
+        m = 64
+        c = E.to_device(mont(x[:m]), F)
+        ev = E.empty_device(4 * m, F)
+        E.coset_lde_device(c, ev, F, m, 1, 4)
+        assert np.array_equal(E.from_device(ev, F), mont(O.coset_lde(F.id, x[:m], 4)))
+        out = E.empty_device(32, F)
+        E.partial_device(E.to_device(mont(x), F), out, F, 32)
+        assert np.array_equal(E.from_device(out, F), mont(O.ntt(F.id, x)[:32]))
+
+
+@pytest.mark.parametrize("fname", ["Goldilocks", "BabyBear"])
+def test_host_staging_large(fname):
+    """Host buffers of 8 MiB and more go through the pinned staging ring
+    (16 MiB slots, the last one ragged); omr_host_alloc buffers go direct.
+    Every form must give the device path's result bit for bit."""
+    F = field(fname)
+    n = 1 << 22  # 32 MiB (Goldilocks) / 16 MiB (BabyBear)
+    x = inputs(F, n, 11)
+    t = E.to_device(x, F)
+    E.ntt_device(t, t, F)
+    want = E.from_device(t, F)
+    y = x.astype(F.np_dtype)
+    E.ArkRadix2().ntt(y, E.NttDomain(n, F))  # pageable: staged
+    assert np.array_equal(y, want)
+    E.intt(y, F)
+    assert np.array_equal(y, x)
+    pinned = E.host_empty(n, F)
+    pinned[:] = x
+    E.ArkRadix2().ntt(pinned, E.NttDomain(n, F))  # pinned: direct
+    assert np.array_equal(pinned, want)
+    # ragged chunking: 5 vectors of 2^20 (40 MiB / 20 MiB)
+    nb, b = 1 << 20, 5
+    xs = inputs(F, nb * b, 12)
+    ts = E.to_device(xs, F)
+    E.ntt_device(ts, ts, F, batch=b, n=nb)
+    ys = xs.copy()
+    E.ntt_batched(ys, F, b)
+    assert np.array_equal(ys, E.from_device(ts, F))
+    # coset LDE of 4 columns of 2^20 (out 4x larger) and its decode
+    m, cols = 1 << 20, 4
+    c = inputs(F, m * cols, 13)
+    ev = E.coset_lde(c, F, m, cols, 4)
+    dc, dev = E.to_device(c, F), E.empty_device(m * cols * 4, F)
+    E.coset_lde_device(dc, dev, F, m, cols, 4)
+    assert np.array_equal(ev, E.from_device(dev, F))
+    back, nz = E.coset_decode(ev, F, m, cols, 4)
+    assert np.array_equal(back, c) and nz == 0
 
 
 def test_encoder_registry_names():

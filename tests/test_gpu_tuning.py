@@ -96,9 +96,11 @@ def test_learned_schedules_regret_and_gain():
     shapes = CM.shape_grid(fields=[E.Goldilocks, E.BabyBear], log_ns=range(10, 23), batches=(1, 8),
                            partial=True, max_elems=1 << 25)
     res, model = CM.experiment(shapes, reps=3, seed=0, steps=3000)
-    os.makedirs("gpurun_out", exist_ok=True)
-    with open("gpurun_out/costmodel_regret.json", "w") as fh:
-        json.dump(res, fh, indent=1)
+    out_dir = os.environ.get("OMR_RESULTS_DIR")  # where to keep the regret table, if set
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        with open(os.path.join(out_dir, "costmodel_regret.json"), "w") as fh:
+            json.dump(res, fh, indent=1)
     print("regret model %.4f default %.4f within10 model %.2f default %.2f" % (
         res["heldout_mean_regret_model"], res["heldout_mean_regret_default"],
         res["heldout_frac_within_10pct_model"], res["heldout_frac_within_10pct_default"]))
