@@ -130,8 +130,11 @@ omr_status omr_default_schedule(int field, uint64_t n, int *log_radix, int *npas
  * reference's `out[0..min(2s, N)]` with out_len = min(2s, N).  mode 1
  * (stride subset): out[j] = X[j * n / out_len] for j < out_len, out_len a
  * power of two dividing n.  method: 0 auto, 1 direct partial DFT,
- * 2 decomposition with sub-NTT size 2^log_n1 (0 = auto), 3 full transform
- * then prefix.  d_out holds out_len elements. */
+ * 2 decomposition with sub-NTT size 2^log_n1 (0 = auto; with out_len <= n1
+ * <= n/4 the sub-NTTs run in passes of up to 2^10 with the recombine fused
+ * into the last, else n1 is at most the largest pass radix, 2^12 for
+ * Goldilocks and 2^13 for the 32-bit fields), 3 full transform then prefix.
+ * d_out holds out_len elements. */
 omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_in, void *d_out, uint64_t n,
                               uint64_t omega, uint64_t out_len, int mode, int method,
                               uint32_t log_n1, void *stream);
@@ -239,8 +242,8 @@ omr_status omr_host_free(void *p);
  * shape is (field, n, out_len, batch); candidate schedules come from a fixed
  * vocabulary of classes: 0..29 full transforms (pass count, pass-radix split,
  * register radix), 30..35 prefix methods (direct, decomposition with n1 =
- * 2^6 / 2^8 / 2^10 / largest one-pass radix, full transform with a pruned
- * last pass).  Tuned choices form a process-wide table consulted by every
+ * 2^6 / 2^8 / 2^10 / 4 out_len in several fused passes, full transform with
+ * a pruned last pass).  Tuned choices form a process-wide table consulted by every
  * transform; the first context loads tuning/mi355x.tune beside the library
  * ($OMR_TUNE_FILE overrides, "none" disables). */
 int omr_num_schedule_classes(void);

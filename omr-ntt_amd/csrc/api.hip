@@ -605,9 +605,11 @@ extern "C" omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const
   if (variant == OMR_DFT_PARTIAL) method = 1;
   else {
     // reference rule: n1 = nextpow2(2 out_len) clamped to [2, N]
-    // (transform_decomposition.rs:38-39); capped at the largest one-pass radix
+    // (transform_decomposition.rs:38-39); where the sub-NTTs cannot run as
+    // fused passes (n1 > N/4), capped at the largest pass radix
     uint64_t n1 = std::min<uint64_t>(std::max<uint64_t>(next_pow2(2 * out_len), 2), n);
-    log_n1 = std::min<uint32_t>(ilog2(n1), (uint32_t)max_log_radix(es));
+    log_n1 = ilog2(n1);
+    if (!decomposition_ok(es, logN, log_n1, out_len)) log_n1 = std::min<uint32_t>(log_n1, (uint32_t)max_log_radix(es));
   }
   for (uint64_t b = 0; b < batch; b++) {
     const char *in = (const char *)d_in + b * n * es;
@@ -679,7 +681,7 @@ extern "C" omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_
   if (mode == 1 && (out_len & (out_len - 1))) return OMR_E_OUTLEN;
   if (method < 0 || method > 3) return OMR_E_ARG;
   const int es = field_bytes(field);
-  if (method == 2 && log_n1 != 0 && (log_n1 > ilog2(n) || (int)log_n1 > max_log_radix(es)))
+  if (mode == 0 && method == 2 && log_n1 != 0 && !decomposition_ok(es, ilog2(n), log_n1, out_len))
     return OMR_E_SCHEDULE;
   if (!ctx || !d_in || !d_out) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(ctx->mu);

@@ -51,10 +51,11 @@ enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u };
 //   kEpiScale   -- kTwiddle, every output times scale_w (inverse transforms)
 //   kEpiGeneral -- everything: zero tail, pre/post multipliers, scale,
 //                  store_limit pruning and the nonzero count
-//   kEpiPrefix  -- (kPrefixSum, the first pass s = N/R of a prefix-partial
-//                  transform) output row k < store_limit of column q times
-//                  w_N^(q k), summed over the tile's columns: one partial sum
-//                  per (k, tile) instead of the pass output
+//   kEpiPrefix  -- (kPrefixSum, the last pass run of a prefix-partial
+//                  transform, s >= C) output k' < store_limit of column
+//                  j = q mod s times w_N^(j k'), summed over the tile's
+//                  columns: one partial sum per (k', tile) instead of the
+//                  pass output
 enum : int { kEpiPlain = 0, kEpiScale = 1, kEpiGeneral = 2, kEpiPrefix = 3 };
 constexpr int kPrefixMaxLogR = 10;  // radices compiled with the prefix epilogue
 
@@ -266,6 +267,9 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     const uint32_t tile = (uint32_t)(gt & ((1ull << logTiles) - 1));
     const T *__restrict__ src = (const T *)a.src + vec * a.vec_stride_in;
     T *__restrict__ dst = (T *)a.dst + vec * a.vec_stride_out;
+    if constexpr (EPI == kEpiPrefix) {  // tiles whose p has no k' = p + P k < out_len
+      if (((tile << logC) >> logS) >= (uint32_t)a.store_limit) return;
+    }
     T x[E];
     load_tile<F, LOGR, LOGE, EPI>(x, src, a, tile, tid, c, t, staged);
 
@@ -329,38 +333,43 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
     constexpr int EL = 1 << K::LOGLAST;
     if constexpr (EPI == kEpiPrefix) {
-      // With s = N/R, p = 0 and output row k of column q is
-      // Y[q][k] = NTT_R(x[q :: N/R])[k]; the prefix X[k] = sum_q w_N^(q k) Y[q][k]
-      // (the decomposition of transform_decomposition.rs:63-94 with n1 = R).
-      // Per tile (columns q = q0 + c): w_N^(q0 k) sum_c w_N^(c k) Y[q0 + c][k],
-      // the inner exponents < C R from the lo table (one lookup when C R fits).
-      const uint32_t out_len = (uint32_t)a.store_limit;  // <= R
+      // Output row k of column q = p s + j (j < s) holds
+      // Z_j[k'] = sum_a x[j + s a] w_(N/s)^(a k'), k' = p + P k, P = N/(R s):
+      // the N/s-point transform of x[j :: s], whose last digit this pass
+      // added (the first pass, s = N/R, has P = 1, p = 0, Z_j = NTT_R(x[j :: N/R])).
+      // The prefix is X[k'] = sum_j w_N^(j k') Z_j[k'] for k' < out_len <= N/s
+      // (the decomposition of transform_decomposition.rs:63-94 with n1 = N/s).
+      // A tile holds one p and the columns j = j0 + c (C <= s):
+      // w_N^(j0 k') sum_c w_N^(c k') Z_(j0 + c)[k'], one partial sum per (k', tile).
+      const uint32_t out_len = (uint32_t)a.store_limit;  // <= N/s
+      const uint32_t logP = a.logN - LOGR - logS;
+      const uint32_t nrows = min((uint32_t)R, (out_len - p + (1u << logP) - 1) >> logP);  // k' < out_len
       const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
       const uint32_t lomask = (1u << a.lo_bits) - 1;
-      const bool one = (C << LOGR) <= (1u << a.lo_bits);
+      const bool one = ((C << LOGR) << logP) <= (1u << a.lo_bits);  // c k' < C N/s
       __syncthreads();  // the last in-tile exchange has been read
 #pragma unroll
       for (int g = 0; g < E / EL; g++) {
 #pragma unroll
         for (int kk = 0; kk < EL; kk++) {
           const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
-          if (krow < out_len) {
-            const uint32_t e = krow * c;  // < C R
+          if (krow < nrows) {
+            const uint32_t e = c * (p + (krow << logP));
             const T w = one ? tlo[e] : F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]);
             lds[lpad((krow << logC) + c)] = F::mulw(x[g * EL + kk], w);
           }
         }
       }
       __syncthreads();
-      // partial sums laid out [vector][k][tile]: the reduction over tiles reads
-      // contiguous runs
-      const uint64_t tpv = 1ull << logTiles;
-      T *__restrict__ part = (T *)a.dst + (gt >> logTiles) * tpv * out_len + (gt & (tpv - 1));
-      const uint32_t q0 = tile << logC;
+      // partial sums laid out [vector][k'][tile of this p]: the reduction over
+      // tiles reads contiguous runs
+      const uint32_t j0 = (tile << logC) & (S - 1);
+      const uint64_t tps = (uint64_t)S >> logC;  // tiles per p
+      T *__restrict__ part = (T *)a.dst + (gt >> logTiles) * tps * out_len + (j0 >> logC);
       const bool big = a.logN > a.log_mid;
       const T *tab = a.logN <= a.log_rmax ? root : (const T *)a.tw_mid;
       const uint32_t sh = (a.logN <= a.log_rmax ? a.log_rmax : a.log_mid) - a.logN;
-      for (uint32_t r = tid; r < out_len; r += nthr) {
+      for (uint32_t r = tid; r < nrows; r += nthr) {
         T acc[4] = {0, 0, 0, 0};  // independent chains over the columns
         uint32_t cc = 0;
         for (; cc + 4 <= C; cc += 4) {
@@ -369,9 +378,10 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
         }
         for (; cc < C; cc++) acc[0] = F::add(acc[0], lds[lpad((r << logC) + cc)]);
         const T sum = F::add(F::add(acc[0], acc[1]), F::add(acc[2], acc[3]));
-        const uint32_t e = r * q0;  // < N
+        const uint32_t k = p + (r << logP);
+        const uint32_t e = k * j0;  // < N
         const T w = big ? F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]) : tab[e << sh];
-        part[(uint64_t)r * tpv] = F::mulw(sum, w);
+        part[(uint64_t)k * tps] = F::mulw(sum, w);
       }
       return;
     }

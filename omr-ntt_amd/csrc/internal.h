@@ -125,6 +125,8 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
 omr_status host_to_device(omr_ctx *ctx, void *dev, const void *host, size_t bytes);
 omr_status device_to_host(omr_ctx *ctx, void *host, const void *dev, size_t bytes);
 void free_stage(HostStage &s);
+uint32_t multipass_log_n1(uint32_t logN, uint64_t out_len);
+bool decomposition_ok(int es, uint32_t logN, uint32_t log_n1, uint64_t out_len);
 omr_status lde_args(int field, uint64_t n, uint64_t ncols, uint32_t blowup, uint64_t &shift, uint32_t &logM);
 omr_status tune_classes_subset(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len, uint64_t batch, int reps,
                                const bool *only, float *times_us);

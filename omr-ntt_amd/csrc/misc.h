@@ -158,6 +158,26 @@ __global__ void __launch_bounds__(256) k_sum_contig(const typename F::T *__restr
   if (threadIdx.x == 0) out[(uint64_t)blockIdx.y * os + blockIdx.x] = red[0];
 }
 
+// The same sums when each is short (n <= 32): one thread per sum, grid
+// (ceil(m / 256), nvec); four independent chains over t.
+template <class F>
+__global__ void __launch_bounds__(256) k_sum_short(const typename F::T *__restrict__ part,
+                                                   typename F::T *__restrict__ out, uint32_t n, uint64_t m,
+                                                   uint64_t os) {
+  using T = typename F::T;
+  const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= m) return;
+  const T *row = part + ((uint64_t)blockIdx.y * m + k) * n;
+  T acc[4] = {0, 0, 0, 0};
+  uint32_t t = 0;
+  for (; t + 4 <= n; t += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[u] = F::add(acc[u], row[t + u]);
+  }
+  for (; t < n; t++) acc[0] = F::add(acc[0], row[t]);
+  out[(uint64_t)blockIdx.y * os + k] = F::add(F::add(acc[0], acc[1]), F::add(acc[2], acc[3]));
+}
+
 // Elementwise field ops on canonical operands (device arithmetic unit test):
 // op 0 add, 1 sub, 2 mul, 3 to_w then from_w round trip of a, 4 mulw(a, to_w(b)).
 template <class F>

@@ -111,8 +111,12 @@ omr_status launch_sum_contig(int f, const void *part, void *out, uint64_t n, uin
                              uint64_t os, hipStream_t s) {
   OMR_FIELD_SWITCH(f, {
     using T = typename F::T;
-    hipLaunchKernelGGL(k_sum_contig<F>, dim3((uint32_t)m, nvec), dim3(256), 0, s, (const T *)part, (T *)out, n, m,
-                       os);
+    if (n <= 32)  // short sums: a thread each
+      hipLaunchKernelGGL(k_sum_short<F>, dim3((uint32_t)((m + 255) / 256), nvec), dim3(256), 0, s,
+                         (const T *)part, (T *)out, (uint32_t)n, m, os);
+    else
+      hipLaunchKernelGGL(k_sum_contig<F>, dim3((uint32_t)m, nvec), dim3(256), 0, s, (const T *)part, (T *)out, n,
+                         m, os);
   });
   return last_launch();
 }

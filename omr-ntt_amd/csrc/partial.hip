@@ -99,50 +99,99 @@ static omr_status fold_rows(omr_ctx *ctx, int f, const void *src, uint64_t rows,
   return OMR_OK;
 }
 
-// Prefix out[k] = X[k], k < out_len <= 2^log_n1, of nvec vectors (vector v at
-// d_in + v in_stride, its prefix at d_out + v out_stride): one radix-2^log_n1
-// pass whose epilogue folds w_N^(q k) Y[q][k] over each tile's columns
-// (kEpiPrefix), then each vector's per-tile partial sums added up.  The input
-// is read once and nothing N-sized is written.
+// Pass radices of the sub-NTTs of size 2^log_n1 in the fused prefix: one
+// pass up to 2^10 (the prefix epilogue's largest radix), else the fewest
+// passes of at most 2^10, balanced, larger ones first.
+static std::vector<int> prefix_split(uint32_t log_n1) {
+  const int np = ((int)log_n1 + kPrefixMaxLogR - 1) / kPrefixMaxLogR;
+  std::vector<int> lr(np, (int)log_n1 / np);
+  for (int i = 0; i < (int)log_n1 % np; i++) lr[i]++;
+  return lr;
+}
+
+// Whether the fused prefix runs for (N, n1): the last pass's tiles must not
+// straddle two values of p (C <= s = N/n1).
+static bool prefix_fused_ok(uint32_t logN, uint32_t log_n1) {
+  if (log_n1 == 0 || log_n1 > logN) return false;
+  const std::vector<int> lr = prefix_split(log_n1);
+  if (lr.size() == 1) return (int)log_n1 <= kPrefixMaxLogR;
+  return logN - log_n1 >= 2;  // s >= 4 columns per tile
+}
+
+// Whether the decomposition with sub-NTT size 2^log_n1 can run a prefix of
+// out_len: fused (out_len <= n1, sub-NTTs in passes of up to 2^10), or one
+// pass of radix n1 and a separate recombine.
+bool decomposition_ok(int es, uint32_t logN, uint32_t log_n1, uint64_t out_len) {
+  if (log_n1 == 0 || log_n1 > logN) return false;
+  if (out_len <= (1ull << log_n1) && prefix_fused_ok(logN, log_n1)) return true;
+  return (int)log_n1 <= max_log_radix(es);
+}
+
+// Prefix out[k] = X[k], k < out_len <= n1 = 2^log_n1, of nvec vectors (vector
+// v at d_in + v in_stride, its prefix at d_out + v out_stride): the first
+// passes of an N-point transform covering log n1 input digits (their
+// outputs are the n1-point sub-NTTs of x[j :: N/n1]), the last of them with
+// the prefix epilogue folding w_N^(j k) Z_j[k] over each tile's columns
+// (kEpiPrefix), then each vector's per-tile partial sums added up.  With
+// one pass the input is read once and nothing N-sized is written.
 static omr_status prefix_fused(omr_ctx *ctx, int f, const void *d_in, uint64_t in_stride, void *d_out,
                                uint64_t out_stride, uint32_t logN, uint64_t omega, uint64_t out_len,
                                uint32_t log_n1, uint32_t nvec, hipStream_t stream) {
   const int es = field_bytes(f);
+  const uint64_t N = 1ull << logN;
   const TwTables *tab;
   omr_status st = get_tables(ctx, f, logN, omega, &tab);
   if (st != OMR_OK) return st;
-  const int logR = (int)log_n1;
-  const int logE = logR <= 2 ? logR : std::min(4, logR);
-  const int logC = pass_log_cols((int)logN, logR, logE, es);
-  const uint64_t tpv = ((1ull << logN) >> logR) >> logC;  // tiles per vector
-  if ((st = ensure_buf(ctx->scratch[2], (size_t)nvec * tpv * out_len * es)) != OMR_OK) return st;
-  PassArgs a{};
-  a.src = d_in;
-  a.dst = ctx->scratch[2].p;
-  a.tw_lo = tab->lo;
-  a.tw_hi = tab->hi;
-  a.root = tab->root;
-  a.lo_bits = tab->lo_bits;
-  a.root_shift = tab->log_rmax - logR;
-  a.tw_mid = tab->mid;
-  a.log_rmax = tab->log_rmax;
-  a.log_mid = tab->log_mid;
-  a.logN = logN;
-  a.logS = logN - logR;
-  a.logC = logC;
-  a.vec_stride_in = in_stride;
-  a.vec_stride_out = 0;
-  a.store_limit = out_len;
-  a.flags = kPrefixSum;
-  a.nvec = nvec;
-  dim3 block((1u << logC) << (logR - logE));
-  if (launch_pass(f, logR, logE, tab->unit(), a, tpv * nvec, block, stream) != hipSuccess) return OMR_E_DEVICE;
+  const std::vector<int> lr = prefix_split(log_n1);
+  const int np = (int)lr.size();
+  const uint32_t logS_last = logN - log_n1;
+  if (np >= 2) {
+    if ((st = ensure_buf(ctx->scratch[0], (size_t)nvec * N * es)) != OMR_OK) return st;
+    if (np >= 3 && (st = ensure_buf(ctx->scratch[1], (size_t)nvec * N * es)) != OMR_OK) return st;
+  }
+  const int logR_last = lr.back();
+  const int logE_last = logR_last <= 2 ? logR_last : std::min(4, logR_last);
+  const int logC_last = std::min<int>(pass_log_cols((int)logN, logR_last, logE_last, es), (int)logS_last);
+  const uint64_t tps = (1ull << logS_last) >> logC_last;  // last-pass tiles per value of p
+  if ((st = ensure_buf(ctx->scratch[2], (size_t)nvec * tps * out_len * es)) != OMR_OK) return st;
+  const void *src = d_in;
+  uint32_t consumed = 0;
+  for (int i = 0; i < np; i++) {
+    const int logR = lr[i];
+    consumed += logR;
+    const bool last = i == np - 1;
+    const int logE = logR <= 2 ? logR : std::min(4, logR);
+    const int logC = last ? logC_last : pass_log_cols((int)logN, logR, logE, es);
+    PassArgs a{};
+    a.src = src;
+    a.dst = last ? ctx->scratch[2].p : ctx->scratch[i % 2].p;
+    a.tw_lo = tab->lo;
+    a.tw_hi = tab->hi;
+    a.root = tab->root;
+    a.lo_bits = tab->lo_bits;
+    a.root_shift = tab->log_rmax - logR;
+    a.tw_mid = tab->mid;
+    a.log_rmax = tab->log_rmax;
+    a.log_mid = tab->log_mid;
+    a.logN = logN;
+    a.logS = logN - consumed;
+    a.logC = logC;
+    a.vec_stride_in = i == 0 ? in_stride : N;
+    a.vec_stride_out = last ? 0 : N;
+    a.store_limit = last ? out_len : ~0ull;
+    a.flags = (i > 0 ? kTwiddle : 0u) | (last ? kPrefixSum : 0u);
+    a.nvec = nvec;
+    dim3 block((1u << logC) << (logR - logE));
+    const uint64_t tiles = ((N >> logR) >> logC) * nvec;
+    if (launch_pass(f, logR, logE, tab->unit(), a, tiles, block, stream) != hipSuccess) return OMR_E_DEVICE;
+    src = a.dst;
+  }
   // partials [vector][k][tile]: one block per (k, vector) sums a vector's tiles
   constexpr uint32_t kMaxY = 65535;
   for (uint32_t v0 = 0; v0 < nvec; v0 += kMaxY) {
     const uint32_t nv = std::min(kMaxY, nvec - v0);
-    if ((st = launch_sum_contig(f, (const char *)ctx->scratch[2].p + (size_t)v0 * tpv * out_len * es,
-                                (char *)d_out + (size_t)v0 * out_stride * es, tpv, out_len, nv, out_stride,
+    if ((st = launch_sum_contig(f, (const char *)ctx->scratch[2].p + (size_t)v0 * tps * out_len * es,
+                                (char *)d_out + (size_t)v0 * out_stride * es, tps, out_len, nv, out_stride,
                                 stream)) != OMR_OK)
       return st;
   }
@@ -155,6 +204,18 @@ static uint32_t prefix_log_n1(uint32_t logN, uint64_t out_len) {
   uint32_t l = 6;
   while ((1ull << l) < out_len) l++;
   return std::min<uint32_t>(std::min<uint32_t>(l, (uint32_t)kPrefixMaxLogR), logN);
+}
+
+// Sub-NTT size of the multi-pass fused prefix for out_len > 2^10: four times
+// out_len, at most N/64 (the last pass keeps 64 columns per value of p), or
+// 0 where that is below out_len (the full transform with a pruned last pass
+// is then faster: N = 2^22, out_len >= 2^18, measured).
+uint32_t multipass_log_n1(uint32_t logN, uint64_t out_len) {
+  uint32_t lo = 0;
+  while ((1ull << lo) < out_len) lo++;
+  if (logN < 6) return 0;
+  const uint32_t l = std::min<uint32_t>(lo + 2, logN - 6);
+  return (l >= lo && l > (uint32_t)kPrefixMaxLogR) ? l : 0;
 }
 
 omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
@@ -196,7 +257,7 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
     } else if (out_len <= (1ull << kPrefixMaxLogR)) {
       method = 2;
       log_n1 = prefix_log_n1(logN, out_len);
-    } else if (out_len <= (1ull << 13)) {
+    } else if ((log_n1 = multipass_log_n1(logN, out_len)) != 0) {
       method = 2;
     } else {
       method = 3;
@@ -217,10 +278,11 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
   }
   if (method == 2) {
     if (log_n1 == 0) log_n1 = std::min<uint32_t>(logN, (uint32_t)maxlr);
-    if (log_n1 == 0 || (int)log_n1 > maxlr || log_n1 > logN) return OMR_E_SCHEDULE;
+    if (log_n1 == 0 || log_n1 > logN) return OMR_E_SCHEDULE;
     const uint64_t n1 = 1ull << log_n1, n2 = N >> log_n1;
-    if (out_len <= n1 && (int)log_n1 <= kPrefixMaxLogR)  // recombine fused into the pass
+    if (out_len <= n1 && prefix_fused_ok(logN, log_n1))  // recombine fused into the last pass
       return prefix_fused(ctx, f, d_in, N, d_out, out_len, logN, omega, out_len, log_n1, 1, stream);
+    if ((int)log_n1 > maxlr) return OMR_E_SCHEDULE;  // one pass computes the sub-NTTs
     const uint64_t rows = std::min<uint64_t>(out_len, n1);
     if ((st = ensure_buf(ctx->scratch[2], (size_t)rows * n2 * es)) != OMR_OK) return st;
     if ((st = first_pass(ctx, f, d_in, ctx->scratch[2].p, logN, (int)log_n1, omega, rows * n2,
@@ -256,7 +318,10 @@ omr_status partial_batch_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out
   if (out_len <= (1ull << kPrefixMaxLogR))
     return prefix_fused(ctx, f, d_in, N, d_out, out_len, logN, omega, out_len, prefix_log_n1(logN, out_len),
                         (uint32_t)batch, stream);
-  // longer prefixes: the full transform of every vector, its last pass storing k < out_len
+  const uint32_t l = multipass_log_n1(logN, out_len);
+  if (l != 0 && batch <= 0xFFFFFFFFull)
+    return prefix_fused(ctx, f, d_in, N, d_out, out_len, logN, omega, out_len, l, (uint32_t)batch, stream);
+  // long prefixes: the full transform of every vector, its last pass storing k < out_len
   Schedule sch;
   if (!tuned_schedule(f, logN, sch)) default_schedule(f, logN, sch);
   XformOpts o;

@@ -11,8 +11,9 @@
 //                     register radix 2^(2 + (c-3) % 3)
 //   partial prefix (out_len < N): classes 30..35
 //     30: direct partial DFT; 31..33: decomposition with n1 = 2^6, 2^8,
-//     2^10; 34: decomposition with the largest one-pass radix (2^12, or
-//     2^13 for 32-bit fields, at most N); 35: full transform, pruned last pass.
+//     2^10; 34: decomposition with n1 = 4 out_len (at most N/64, above 2^10),
+//     its sub-NTTs in several passes with the recombine fused into the last;
+//     35: full transform, pruned last pass.
 // The timing autotuner measures every valid class of a shape; its best is
 // the reference the cost model's regret is measured against.
 #include <hip/hip_runtime.h>
@@ -111,11 +112,15 @@ bool class_partial(int f, int cls, uint32_t logN, uint32_t logOut, int &method, 
     method = 1;
     return logOut <= 6;  // out_len * N products: only tiny prefixes
   }
-  if (c <= 4) {
-    static const int n1s[4] = {6, 8, 10, 12};
-    int ln1 = n1s[c - 1];
-    if (c == 4) ln1 = std::min(maxlr, (int)logN);  // the largest one-pass radix
-    if (ln1 > (int)logN) return false;
+  if (c == 4) {  // multi-pass fused decomposition
+    method = 2;
+    log_n1 = multipass_log_n1(logN, 1ull << logOut);
+    return log_n1 != 0;
+  }
+  if (c <= 3) {
+    static const int n1s[3] = {6, 8, 10};
+    const int ln1 = n1s[c - 1];
+    if (ln1 > (int)logN || ln1 > maxlr) return false;
     method = 2;
     log_n1 = (uint32_t)ln1;
     // recombine work out_len * N / n1 must stay below ~4 N
@@ -266,10 +271,14 @@ extern "C" int omr_default_class(int field, uint64_t n, uint64_t out_len) {
   if (!field_ok(field) || n < 2 || (n & (n - 1)) || out_len == 0 || out_len > n) return -1;
   const uint32_t logN = 63 - __builtin_clzll(n);
   const int maxlr = max_log_radix(field_bytes(field));
-  if (out_len < n) {
+  if (out_len < n) {  // the rule partial_impl applies without a tuned entry
     if (out_len <= 4) return kFullClasses;
-    if (out_len <= (1ull << 13)) return kFullClasses + 4;
-    return kFullClasses + 5;
+    if (out_len <= (1ull << kPrefixMaxLogR)) {  // one fused pass: the class of the next n1 up
+      const uint32_t ln1 = out_len <= 64 ? 6 : (out_len <= 256 ? 8 : 10);
+      if (ln1 <= logN) return kFullClasses + (int)(ln1 - 4) / 2;
+      return kFullClasses + 5;
+    }
+    return multipass_log_n1(logN, out_len) ? kFullClasses + 4 : kFullClasses + 5;
   }
   Schedule d;
   default_schedule(field, logN, d);

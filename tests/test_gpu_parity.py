@@ -252,10 +252,32 @@ def test_partial_prefix_fused(fname):
 
 
 @pytest.mark.parametrize("fname", FIELD_IDS)
+def test_partial_prefix_fused_multipass(fname):
+    """Sub-NTTs of size n1 > 2^10 in two or three passes, the recombine fused
+    into the last one (its tiles carry one value p of the earlier digits,
+    k' = p + P k).  out_len below P (tiles with nothing to add), ragged and
+    power-of-two out_len, s = N/n1 down to 4 columns."""
+    F = field(fname)
+    for logn, cases in ((16, [(11, (50, 64, 2047, 2048)), (12, (50, 100, 4096)), (14, (4095, 16384))]),
+                        (20, [(12, (4096,)), (16, (1000, 65536)), (18, (262144,))]),
+                        (22, [(20, (5000,))])):
+        n = 1 << logn
+        x = inputs(F, n, logn + 3)
+        X = O.ntt(F.id, x)
+        t = E.to_device(x, F)
+        for log_n1, lens in cases:
+            for out_len in lens:
+                out = E.empty_device(out_len, F)
+                E.partial_device(t, out, F, out_len, mode=E.PREFIX, method=2, log_n1=log_n1)
+                assert np.array_equal(E.from_device(out, F), X[:out_len]), (logn, log_n1, out_len)
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
 def test_partial_batch(fname):
     """The OMR epoch: a batch of receivers, each needing k = 2s outputs of its
     own vector (benches/common/mod.rs:10-12; s = 1000 is the reference's
-    default), fused (k <= 1024) and full-transform (k > 1024) paths."""
+    default): fused in one pass (k <= 1024), in two passes (k = 2000 at
+    N = 2^18) and the full transform with a pruned last pass (N = 2^12)."""
     F = field(fname)
     for logn, batch, ks in ((12, 5, (1, 7, 256, 1024, 2000, 4096)), (18, 3, (2000, 200)), (20, 2, (1024,))):
         n = 1 << logn
