@@ -118,6 +118,8 @@ __global__ void __launch_bounds__(256) k_td_recombine(const typename F::T *__res
   if (lane == 0 && k < out_len) part[(uint64_t)blockIdx.y * out_len + k] = red[kl][0];
 }
 
+constexpr int kFoldRows = 32;  // rows per chunk of the row folds
+
 // Row fold: part[c * m + i] = sum_{t in chunk c} x[i + m t] (mod p), grid
 // (ceil(m / 256), nchunks, nvec); chunk c covers t in [c tpc, (c+1) tpc);
 // set z reads x + z xs and writes part + z ps.
@@ -133,8 +135,16 @@ __global__ void __launch_bounds__(256) k_fold(const typename F::T *__restrict__ 
   const uint64_t nt = n / m;
   const uint64_t t0 = (uint64_t)blockIdx.y * tpc;
   const uint64_t t1 = t0 + tpc < nt ? t0 + tpc : nt;
+  const T *__restrict__ col = x + i + m * t0;
   T acc = 0;
-  for (uint64_t t = t0; t < t1; t++) acc = F::add(acc, x[i + m * t]);
+  if (t1 - t0 == kFoldRows) {  // full chunk: every load issued before the adds, four chains
+    T a[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < kFoldRows; r++) a[r & 3] = F::add(a[r & 3], col[(uint64_t)r * m]);
+    acc = F::add(F::add(a[0], a[1]), F::add(a[2], a[3]));
+  } else {
+    for (uint64_t t = 0; t < t1 - t0; t++) acc = F::add(acc, col[m * t]);
+  }
   part[(uint64_t)blockIdx.y * m + i] = acc;
 }
 

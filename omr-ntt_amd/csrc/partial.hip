@@ -77,7 +77,7 @@ static omr_status fold_rows(omr_ctx *ctx, int f, const void *src, uint64_t rows,
   uint64_t cur_stride = src_stride;
   int level = 0;
   do {
-    const uint64_t nch = std::min<uint64_t>((rows + 31) / 32, 65535);
+    const uint64_t nch = std::min<uint64_t>((rows + 31) / 32, 65535);  // 32 rows a chunk: the fold kernel's unrolled case
     void *out;
     uint64_t out_stride;
     if (nch == 1) {

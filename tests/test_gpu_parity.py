@@ -432,6 +432,29 @@ def test_coset_lde_columns(fname):
     assert int(nz.item()) > 0
 
 
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_coset_lde_blowups(fname):
+    """Blowups 2 to 64 and n from 1 up, several columns, default and custom
+    shifts.  The first pass (radix = blowup) is skipped: the second reads the
+    replicated input c[pos mod n] h^(pos mod n)."""
+    F = field(fname)
+    for i, (n, blow, ncols) in enumerate(((1, 4, 3), (2, 2, 2), (8, 8, 2), (256, 16, 3), (1 << 14, 2, 2),
+                                          (1 << 12, 64, 1), (1 << 16, 4, 2))):
+        shift = 0 if i % 2 == 0 else 11
+        x = inputs(F, n * ncols, n + blow)
+        c = E.to_device(x, F)
+        ev = E.empty_device(n * blow * ncols, F)
+        E.coset_lde_device(c, ev, F, n, ncols, blow, shift)
+        got = E.from_device(ev, F).reshape(ncols, n * blow)
+        for col in range(ncols):
+            want = O.coset_lde(F.id, x[col * n:(col + 1) * n], blow, shift if shift else None)
+            assert np.array_equal(got[col], want), (n, blow, ncols, shift, col)
+        back = E.empty_device(n * ncols, F)
+        nz = torch.zeros(1, dtype=torch.int32, device="cuda")
+        E.coset_decode_device(ev, back, F, n, ncols, blow, shift, nonzero=nz)
+        assert torch.equal(back, c) and int(nz.item()) == 0, (n, blow, shift)
+
+
 def test_cpp_reference_test_suite():
     """tests/cpp/test_encoders.cpp: the reference's src/lib.rs tests in C++
     through include/omrntt.hpp (built by __graft_entry__.build)."""
