@@ -124,6 +124,13 @@ void free_buf(DevBuf &b) {
 }
 
 // Two-level power table of `root` (order 2^logN) plus the radix root table.
+void pin_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, int delta) {
+  auto it = ctx->tables.find(std::make_tuple(f, logN, root));
+  if (it == ctx->tables.end()) return;
+  if (delta > 0) it->second.pins++;
+  else if (it->second.pins) it->second.pins--;
+}
+
 omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const TwTables **out) {
   auto key = std::make_tuple(f, logN, root);
   auto it = ctx->tables.find(key);
@@ -131,15 +138,19 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
     *out = &it->second;
     return OMR_OK;
   }
-  if (ctx->tables.size() >= 128) {
+  if (ctx->tables.size() >= 128) {  // bounded cache: drop every entry no plan holds
     if (hipDeviceSynchronize() != hipSuccess) return OMR_E_DEVICE;
-    for (auto &kv : ctx->tables) {
-      (void)hipFree(kv.second.lo);
-      (void)hipFree(kv.second.hi);
-      (void)hipFree(kv.second.root);
-      if (kv.second.mid) (void)hipFree(kv.second.mid);
+    for (auto it2 = ctx->tables.begin(); it2 != ctx->tables.end();) {
+      if (it2->second.pins) {
+        ++it2;
+        continue;
+      }
+      (void)hipFree(it2->second.lo);
+      (void)hipFree(it2->second.hi);
+      (void)hipFree(it2->second.root);
+      if (it2->second.mid) (void)hipFree(it2->second.mid);
+      it2 = ctx->tables.erase(it2);
     }
-    ctx->tables.clear();
   }
   if (alloc_fail_injected()) return OMR_E_ALLOC;
   const int es = field_bytes(f);

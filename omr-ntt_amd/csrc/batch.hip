@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "batch_kernel.h"
@@ -28,6 +29,9 @@ struct omr_batch_plan {
   std::vector<uint32_t> ntasks, grid;
   std::vector<void *> d_tasks, d_vecs;
   void *d_tsets = nullptr;
+  // twiddle tables whose device pointers d_tsets holds: pinned in the
+  // context's cache for the plan's lifetime
+  std::vector<std::tuple<int, uint32_t, uint64_t>> pinned;
   DevBuf scratch[2];
   uint64_t total_elems = 0;
   // graph cache
@@ -89,6 +93,10 @@ extern "C" omr_status omr_batch_set_max_log_radix(int log_radix) {
 extern "C" omr_status omr_batch_plan_destroy(omr_batch_plan *p) {
   if (!p) return OMR_E_NULL;
   (void)hipDeviceSynchronize();
+  if (p->ctx && !p->pinned.empty()) {
+    std::lock_guard<std::mutex> lk(p->ctx->mu);
+    for (auto &k : p->pinned) pin_tables(p->ctx, std::get<0>(k), std::get<1>(k), std::get<2>(k), -1);
+  }
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
@@ -152,6 +160,8 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
       }
       const TwTables *tab;
       if ((st = get_tables(ctx, field, logn, w, &tab)) != OMR_OK) break;
+      pin_tables(ctx, field, logn, w, +1);
+      p->pinned.emplace_back(field, logn, w);
       bt.lo = tab->lo;
       bt.hi = tab->hi;
       bt.root = tab->root;
@@ -286,6 +296,9 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
   }
   p->nlaunch = (uint32_t)p->ntasks.size();
   if (st != OMR_OK) {
+    // the context lock is held here: unpin directly, then free the rest
+    for (auto &k : p->pinned) pin_tables(ctx, std::get<0>(k), std::get<1>(k), std::get<2>(k), -1);
+    p->pinned.clear();
     omr_batch_plan_destroy(p);
     return st;
   }

@@ -27,6 +27,7 @@ struct TwTables {
   uint32_t log_mid = 0;
   uint32_t pow2_unit = 0;  // Goldilocks: 2^pow2_unit = root^(2^logN / M), M = 2^log_m = min(2^logN, 64)
   uint32_t log_m = 0;
+  uint32_t pins = 0;  // holders beyond one call (batch plans): the cache never frees a pinned entry
   // register-DFT constant unit of the pass kernels (0 = table constants)
   int unit() const { return (log_m == 6 && (pow2_unit == 39 || pow2_unit == 153)) ? (int)pow2_unit : 0; }
 };
@@ -99,6 +100,10 @@ omr_status validate_domain(int f, uint64_t n, uint64_t omega);
 omr_status ensure_buf(DevBuf &b, size_t bytes);
 void free_buf(DevBuf &b);
 omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const TwTables **out);
+// Pins (delta +1) or unpins (-1) a cached entry for a holder that keeps its
+// device pointers past the call (batch plans bake them into device tables).
+// Call under ctx->mu.
+void pin_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, int delta);
 
 void default_schedule(int f, uint32_t logN, Schedule &s);
 omr_status variant_schedule(int f, int variant, uint32_t logN, Schedule &s);

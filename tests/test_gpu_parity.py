@@ -508,6 +508,32 @@ def test_batched_uneven_sizes_all_modes(fname):
     plan.close()
 
 
+def test_batch_plan_survives_twiddle_cache_eviction():
+    """A plan bakes twiddle-table pointers into its device task tables; the
+    context's table cache (bounded at 128 entries) must not free them while
+    the plan lives.  Fill the cache with 140 other roots, then replay."""
+    F = E.Goldilocks
+    rng = np.random.default_rng(3)
+    items, total = _batch_items(rng, 12, 4, 14)
+    x = inputs(F, total, 5)
+    t_in = E.to_device(x, F)
+    plan = E.BatchPlan(F, items)
+    first = torch.zeros_like(t_in)
+    plan.execute(t_in, first, "graph")
+    want = E.from_device(first, F)
+    n = 1 << 10
+    w = E.root_of_unity(F, n)
+    y = E.to_device(inputs(F, n, 6), F)
+    out = E.empty_device(n, F)
+    for k in range(1, 2 * 140, 2):  # 140 distinct primitive roots w^k, k odd
+        E.ntt_device(y, out, F, omega=pow(w, k, F.p))
+    for mode in ("graph", "persistent"):
+        again = torch.zeros_like(t_in)
+        plan.execute(t_in, again, mode)
+        assert np.array_equal(E.from_device(again, F), want), mode
+    plan.close()
+
+
 def test_batched_in_place():
     F = E.Goldilocks
     rng = np.random.default_rng(8)
