@@ -146,6 +146,26 @@ def host_path(E, F, n, host_x, np):
     return out
 
 
+def other_fields(E, torch, np):
+    """Forward and inverse 2^24 transforms of the three 32-bit fields
+    (BabyBear, KoalaBear, TeddyBear; src/fields.rs:4-17), three rotating
+    HBM-resident inputs, tuned schedules."""
+    out = {}
+    n = 1 << 24
+    for F in (E.BabyBear, E.KoalaBear, E.TeddyBear):
+        xs = [E.to_device(np.random.default_rng(24 + i).integers(0, F.p, size=n, dtype=np.uint64), F)
+              for i in range(3)]
+        ys = [E.empty_device(n, F) for _ in range(3)]
+        fwd = time_device(lambda i: E.ntt_device(xs[i % 3], ys[i % 3], F), 20, 5, torch)
+        inv = time_device(lambda i: E.intt_device(ys[i % 3], xs[i % 3], F), 20, 5, torch)
+        out[F.name] = {"forward_us": fwd * 1e6, "inverse_us": inv * 1e6, "elems_per_s": n / fwd,
+                       "schedule": E.default_schedule(F, n)}
+        log(f"bench: {F.name} 2^24 forward {fwd * 1e6:.1f} us inverse {inv * 1e6:.1f} us")
+        del xs, ys
+    torch.cuda.empty_cache()
+    return out
+
+
 def config1(E, torch, np):
     """BabyBear 2^16 forward + inverse, latency per transform."""
     F = E.BabyBear
@@ -181,21 +201,24 @@ def config3(E, torch, np):
     return out
 
 
-def config5(E, torch, np):
+def config5(E, torch, np, with_model=True):
     """256 BabyBear columns x 2^20 coefficients -> blowup 4 coset LDE
     (n = 2^22) -> decode, with the schedule chosen by the learned cost model
-    (trained live on autotuner timings); held-out regret reported."""
-    from omr_ntt import costmodel as CM
-    shapes = CM.shape_grid(fields=[E.BabyBear, E.Goldilocks], log_ns=range(12, 23, 2), batches=(1, 16, 64),
-                           partial=False, max_elems=1 << 27)
-    res, model = CM.experiment(shapes, reps=3, seed=0, steps=3000)
+    (trained live on autotuner timings; held-out regret reported), or with
+    the shipped tuned table when with_model is False."""
     F = E.BabyBear
     n, cols, blow = 1 << 20, 256, 4
     m = n * blow
-    # model-pruned search: the fastest of the model's 3 best-predicted classes
-    # (3 timing runs instead of the exhaustive ~13)
-    cls, timed = E.autotune_shape(F, m, m, cols, model=model, use_model=True, top_k=3)
-    sched = E.schedule_class(F, m, m, cls)
+    res, cls, timed, sched = {}, None, 0, None
+    if with_model:
+        from omr_ntt import costmodel as CM
+        shapes = CM.shape_grid(fields=[E.BabyBear, E.Goldilocks], log_ns=range(12, 23, 2),
+                               batches=(1, 16, 64), partial=False, max_elems=1 << 27)
+        res, model = CM.experiment(shapes, reps=3, seed=0, steps=3000)
+        # model-pruned search: the fastest of the model's 3 best-predicted
+        # classes (3 timing runs instead of the exhaustive ~13)
+        cls, timed = E.autotune_shape(F, m, m, cols, model=model, use_model=True, top_k=3)
+        sched = E.schedule_class(F, m, m, cls)
     coeffs = torch.randint(0, F.p, (n * cols,), dtype=torch.int32, device="cuda")
     evals = torch.empty(m * cols, dtype=torch.int32, device="cuda")
     back = torch.empty_like(coeffs)
@@ -203,15 +226,19 @@ def config5(E, torch, np):
     enc = time_device(lambda i: E.coset_lde_device(coeffs, evals, F, n, cols, blow), 5, 1, torch)
     dec = time_device(lambda i: E.coset_decode_device(evals, back, F, n, cols, blow, nonzero=nz), 5, 1, torch)
     ok = bool(torch.equal(back, coeffs)) and int(nz.item()) == 0
-    E.tune_clear()
+    if with_model:
+        E.tune_clear()
     enc_bytes = cols * (n + m) * 4
     log(f"bench: encoder 256x2^20 BabyBear blowup 4: encode {enc * 1e3:.2f} ms decode {dec * 1e3:.2f} ms "
-        f"round trip ok={ok}; class {cls} {sched} ({timed} timed); "
-        f"held-out regret {res['heldout_mean_regret_model']:.3f}")
-    return {"encode_ms": enc * 1e3, "decode_ms": dec * 1e3, "round_trip_ok": ok,
-            "encode_alg_GBps": enc_bytes / enc / 1e9, "schedule_class": cls, "schedule": sched,
-            "schedule_search": f"fastest of the cost model's top 3 ({timed} classes timed)",
-            "cost_model": {k: v for k, v in res.items() if k != "per_shape"}}
+        f"round trip ok={ok}; class {cls} {sched} ({timed} timed)"
+        + (f"; held-out regret {res['heldout_mean_regret_model']:.3f}" if with_model else ""))
+    out = {"encode_ms": enc * 1e3, "decode_ms": dec * 1e3, "round_trip_ok": ok,
+           "encode_alg_GBps": enc_bytes / enc / 1e9}
+    if with_model:
+        out.update({"schedule_class": cls, "schedule": sched,
+                    "schedule_search": f"fastest of the cost model's top 3 ({timed} classes timed)",
+                    "cost_model": {k: v for k, v in res.items() if k != "per_shape"}})
+    return out
 
 
 def batched(E, F, torch, np):
@@ -357,6 +384,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 128 MiB device copy (torch copy_) 10 times, to calibrate "
                          "rocprof FETCH_SIZE / WRITE_SIZE against a kernel of known traffic")
+    ap.add_argument("--extras", default="host,sizes,fields,batched,config1,config3,config5,costmodel",
+                    help="comma-separated legs after the headline (costmodel: train the cost model "
+                         "and pick config 5's schedule with it)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 2^20/2^22 sizes and the batched uneven-size workload")
     ap.add_argument("--summarize-profiles", metavar="DIR",
@@ -468,17 +498,26 @@ def main():
             "utilization": valu_ops * n / VALU_LANE_OPS_PER_S / sec},
         "bit_exact": {"gpu_round_trip": round_trip},
     }
-    if not args.no_extras:
+    legs = set() if args.no_extras else set(args.extras.split(","))
+    if legs:
         del xs, ys, back
         torch.cuda.empty_cache()
+    if "host" in legs:
         result["host_path"] = host_path(E, F, n, host_x, np)
+    if "sizes" in legs:
         result["sizes_us"] = other_sizes(E, F, torch, np)
+    if "fields" in legs:
+        result["fields_2p24"] = other_fields(E, torch, np)
+    if "batched" in legs:
         result["batched"] = batched(E, F, torch, np)
         torch.cuda.empty_cache()
+    if "config1" in legs:
         result["config1_babybear_2p16"] = config1(E, torch, np)
+    if "config3" in legs:
         result["config3_partial_2p22"] = config3(E, torch, np)
         torch.cuda.empty_cache()
-        result["config5_encoder"] = config5(E, torch, np)
+    if "config5" in legs:
+        result["config5_encoder"] = config5(E, torch, np, "costmodel" in legs)
     if not args.no_cpu:
         cb, cpu_out = cpu_baseline(host_x, n)
         result["cpu_baseline"] = cb

@@ -374,19 +374,16 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     for (int i = 1; i < np; i++) X[i] = ((np - i) % 2 == 1) ? ctx->scratch[0].p : ctx->scratch[1].p;
   }
 
-  // a replicated zero tail skips the first pass (it only copies the input)
-  const bool skip0 = o.replicate && o.zero_tail && np >= 2 && sch.logR[0] == (int)(logN - o.log_valid);
   int consumed = 0;
   for (int i = 0; i < np; i++) {
     const int logR = sch.logR[i];
     consumed += logR;
-    if (i == 0 && skip0) continue;
     int logE = sch.logE;
     if (!has_pass_kernel(logR, logE, es)) logE = auto_log_e(logR);
     if (!has_pass_kernel(logR, logE, es)) return OMR_E_SCHEDULE;
     const int logC = pass_log_cols((int)logN, logR, logE, es);
     PassArgs a{};
-    a.src = (i == 1 && skip0) ? X[0] : X[i];
+    a.src = X[i];
     a.dst = (void *)X[i + 1];
     a.tw_lo = tab->lo;
     a.tw_hi = tab->hi;
@@ -399,7 +396,7 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.logN = logN;
     a.logS = logN - consumed;
     a.logC = logC;
-    a.vec_stride_in = (i == 0 || (i == 1 && skip0)) ? in_stride : N;
+    a.vec_stride_in = (i == 0) ? in_stride : N;
     a.vec_stride_out = (i == np - 1) ? out_stride : N;
     a.store_limit = (i == np - 1) ? o.store_limit : ~0ull;
     a.flags = 0;
@@ -418,12 +415,6 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
         a.flags |= kZeroTail;
         a.logValid = o.log_valid;
       }
-      a.pre_lo = o.pre_lo;
-      a.pre_hi = o.pre_hi;
-    }
-    if (i == 1 && skip0) {
-      a.flags |= kReplicate;
-      a.logValid = o.log_valid;
       a.pre_lo = o.pre_lo;
       a.pre_hi = o.pre_hi;
     }

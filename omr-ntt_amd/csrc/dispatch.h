@@ -50,8 +50,7 @@ hipError_t launch_pass(int field, int logR, int logE, int unit, const PassArgs &
 // The cheapest pass-kernel variant that honours every option set in `a`.
 inline int pass_epilogue(const PassArgs &a) {
   if (a.flags & kPrefixSum) return kEpiPrefix;
-  if (a.pre_lo || a.post_lo || a.nonzero_count || (a.flags & (kZeroTail | kReplicate)) ||
-      a.store_limit < (1ull << a.logN))
+  if (a.pre_lo || a.post_lo || a.nonzero_count || (a.flags & kZeroTail) || a.store_limit < (1ull << a.logN))
     return kEpiGeneral;
   return (a.flags & kScale) ? kEpiScale : kEpiPlain;
 }

@@ -35,15 +35,14 @@ struct PassArgs {
   uint64_t scale_w;         // multiplier applied to every output when kScale
   uint64_t store_limit;     // only outputs with index < store_limit are stored
   uint32_t logN, logS, logC, lo_bits, root_shift;
-  uint32_t flags;     // kTwiddle | kScale | kZeroTail | kPrefixSum | kReplicate
-  uint32_t logValid;  // kZeroTail: input positions >= 2^logValid read as zero;
-                      // kReplicate: position pos reads pos mod 2^logValid
+  uint32_t flags;     // kTwiddle | kScale | kZeroTail
+  uint32_t logValid;  // with kZeroTail: input positions >= 2^logValid read as zero
   uint32_t nvec;              // vectors (consecutive, vec_stride_in/out apart)
   const void *tw_mid;         // w_N^(j N/2^log_mid), j < 2^log_mid (single-lookup twiddles)
   uint32_t log_rmax;          // root table holds root_(2^log_rmax)^m
   uint32_t log_mid;
 };
-enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u, kReplicate = 16u };
+enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u };
 
 // Pass-kernel variants by the options they honour (chosen per launch): the
 // options' per-element tests, if compiled in, cost VALU and split the store
@@ -231,10 +230,6 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
       const uint32_t pos = pos0 + (uint32_t)m * step;
       xn[m] = ((uint64_t)pos < valid) ? src[pos] : (T)0;
     }
-  } else if (EPI == kEpiGeneral && (a.flags & kReplicate)) {
-    const uint32_t vmask = (uint32_t)((1ull << a.logValid) - 1);
-#pragma unroll
-    for (int m = 0; m < K::E; m++) xn[m] = src[(pos0 + (uint32_t)m * step) & vmask];
   } else {
 #pragma unroll
     for (int m = 0; m < K::E; m++) xn[m] = src[pos0 + (uint32_t)m * step];
@@ -301,10 +296,9 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     if (EPI == kEpiGeneral && a.pre_lo) {  // h^pos on input position pos (coset shift)
       const T *plo = (const T *)a.pre_lo, *phi = (const T *)a.pre_hi;
       const uint32_t lomask = (1u << a.lo_bits) - 1;
-      const uint32_t vmask = (a.flags & kReplicate) ? (uint32_t)((1ull << a.logValid) - 1) : ~0u;
 #pragma unroll
       for (int m = 0; m < E; m++) {
-        const uint32_t pos = (pos0 + ((t + m * RE) << logS)) & vmask;
+        const uint32_t pos = pos0 + ((t + m * RE) << logS);
         x[m] = F::mulw(x[m], F::wmul(plo[pos & lomask], phi[pos >> a.lo_bits]));
       }
     }

@@ -45,10 +45,6 @@ struct XformOpts {
   uint64_t scale_w = 0;         // multiplier form
   bool zero_tail = false;       // input positions >= 2^log_valid read as zero (pass 0)
   uint32_t log_valid = 0;
-  // with zero_tail and a first pass of radix 2^(logN - log_valid): that pass
-  // only copies the (pre-multiplied) input to each of its output rows, so it
-  // is skipped and the second pass reads position pos as in[pos mod 2^log_valid]
-  bool replicate = false;
   const void *pre_lo = nullptr;  // input-position multiplier tables (pass 0), or null
   const void *pre_hi = nullptr;
   uint64_t store_limit = ~0ull;  // only outputs with index < store_limit are stored

@@ -3,10 +3,8 @@
 // The reference has no encoder beyond the forward NTT (src/decode.rs:1 is a
 // stub); these are the closed forms the engine defines:
 //   encode: evals[i] = sum_{j<n} c[j] (h w_m^i)^j, i < m = n * blowup,
-//           = NTT_m(c[j] h^j zero-padded).  Its first pass, of radix
-//           blowup over the input digits only the first value of which is
-//           nonzero, copies c[j] h^j to every output row: it is skipped, and
-//           the second pass reads position pos as c[pos mod n] h^(pos mod n);
+//           = NTT_m(c[j] h^j zero-padded): h^j is fused into the first pass
+//           and the zero tail is never read from HBM;
 //   decode: c[j] = h^-j m^-1 sum_i evals[i] w_m^(-ij) for j < n; the last
 //           pass fuses h^-j m^-1, stores j < n and counts nonzero j >= n
 //           (a codeword has none).
@@ -68,25 +66,11 @@ extern "C" omr_status omr_coset_lde_device(omr_ctx *ctx, int field, const void *
   const TwTables *tw, *sh;
   if ((st = get_tables(ctx, field, logM, w, &tw)) != OMR_OK) return st;
   if ((st = get_tables(ctx, field, logM, shift, &sh)) != OMR_OK) return st;
+  Schedule s;
+  if (!tuned_schedule(field, logM, s)) default_schedule(field, logM, s);
   XformOpts o;
   o.zero_tail = true;
   o.log_valid = ilog2v(n);
-  Schedule s;
-  const uint32_t logB = logM - o.log_valid;
-  Schedule rest;
-  if (logB >= 1 && (int)logB <= max_log_radix(field_bytes(field)) && o.log_valid >= 1) {
-    // first pass of radix B over the zero tail: skipped (replicated input),
-    // the rest run the n-point transform's schedule at its strides
-    if (!tuned_schedule(field, o.log_valid, rest)) default_schedule(field, o.log_valid, rest);
-  }
-  if (!rest.logR.empty() && rest.logR.size() < 8) {
-    s.logR.push_back((int)logB);
-    s.logR.insert(s.logR.end(), rest.logR.begin(), rest.logR.end());
-    s.logE = rest.logE;
-    o.replicate = true;
-  } else if (!tuned_schedule(field, logM, s)) {
-    default_schedule(field, logM, s);
-  }
   if (shift != 1) {
     o.pre_lo = sh->lo;
     o.pre_hi = sh->hi;
