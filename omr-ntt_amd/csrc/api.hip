@@ -131,6 +131,42 @@ void pin_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, int delta) {
   else if (it->second.pins) it->second.pins--;
 }
 
+// hi table of the entry (f, logN, root) times the canonical scale sc, built
+// on first use and kept with the entry (rebuilt if the scale changes).
+static const void *scaled_hi(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, uint64_t sc) {
+  auto it = ctx->tables.find(std::make_tuple(f, logN, root));
+  if (it == ctx->tables.end()) return nullptr;
+  TwTables &t = it->second;
+  if (t.hi_scaled && t.hi_scale == sc) return t.hi_scaled;
+  const int es = field_bytes(f);
+  const uint64_t nhi = 1ull << (logN - t.lo_bits);
+  const uint64_t step = fpow_host(f, root, 1ull << t.lo_bits);
+  std::vector<uint64_t> v(nhi);
+  std::vector<uint32_t> v32;
+  uint64_t cur = sc % kP[f];
+  for (uint64_t j = 0; j < nhi; j++) {
+    v[j] = to_w_host(f, cur);
+    cur = fmul_host(f, cur, step);
+  }
+  const void *src = v.data();
+  if (es == 4) {
+    v32.assign(v.begin(), v.end());
+    src = v32.data();
+  }
+  if (!t.hi_scaled) {
+    if (hipMalloc(&t.hi_scaled, nhi * es) != hipSuccess) {
+      (void)hipGetLastError();
+      t.hi_scaled = nullptr;
+      return nullptr;
+    }
+  } else if (hipDeviceSynchronize() != hipSuccess) {  // the old scale may still be read
+    return nullptr;
+  }
+  if (hipMemcpy(t.hi_scaled, src, nhi * es, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  t.hi_scale = sc;
+  return t.hi_scaled;
+}
+
 omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const TwTables **out) {
   auto key = std::make_tuple(f, logN, root);
   auto it = ctx->tables.find(key);
@@ -149,6 +185,7 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
       (void)hipFree(it2->second.hi);
       (void)hipFree(it2->second.root);
       if (it2->second.mid) (void)hipFree(it2->second.mid);
+      if (it2->second.hi_scaled) (void)hipFree(it2->second.hi_scaled);
       it2 = ctx->tables.erase(it2);
     }
   }
@@ -402,7 +439,14 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.flags = 0;
     if (i > 0) a.flags |= kTwiddle;
     if (i == np - 1) {
-      if (o.scale) {
+      // the last pass (s = 1) multiplies every input by a two-level twiddle
+      // when N > 2^log_mid: the scale rides in a scaled hi table instead
+      const void *shi = (o.scale && o.scale_c && i > 0 && logN > tab->log_mid)
+                            ? scaled_hi(ctx, f, logN, root, o.scale_c)
+                            : nullptr;
+      if (shi) {
+        a.tw_hi = shi;
+      } else if (o.scale) {
         a.flags |= kScale;
         a.scale_w = o.scale_w;
       }
@@ -480,6 +524,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
     (void)hipFree(kv.second.hi);
     (void)hipFree(kv.second.root);
     if (kv.second.mid) (void)hipFree(kv.second.mid);
+    if (kv.second.hi_scaled) (void)hipFree(kv.second.hi_scaled);
   }
   for (auto &b : ctx->scratch) free_buf(b);
   free_buf(ctx->io[0]);
@@ -643,7 +688,8 @@ extern "C" omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in,
   if (!tuned_schedule(field, logN, s)) default_schedule(field, logN, s);
   XformOpts o;
   o.scale = true;
-  o.scale_w = to_w_host(field, finv_host(field, n % kP[field]));
+  o.scale_c = finv_host(field, n % kP[field]);
+  o.scale_w = to_w_host(field, o.scale_c);
   return run_transform(ctx, field, d_in, d_out, logN, finv_host(field, omega), s, batch, n, n, o,
                        (hipStream_t)stream);
 }
@@ -677,7 +723,8 @@ extern "C" omr_status omr_ntt_schedule_device(omr_ctx *ctx, int field, const voi
   if (inverse) {
     root = finv_host(field, omega);
     o.scale = true;
-    o.scale_w = to_w_host(field, finv_host(field, n % kP[field]));
+    o.scale_c = finv_host(field, n % kP[field]);
+    o.scale_w = to_w_host(field, o.scale_c);
   }
   return run_transform(ctx, field, d_in, d_out, logN, root, s, batch, n, n, o, (hipStream_t)stream);
 }

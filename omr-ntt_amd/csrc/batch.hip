@@ -389,7 +389,8 @@ extern "C" omr_status omr_batch_execute_plain(omr_batch_plan *p, const void *d_i
     if (it.inverse) {
       w = finv_host(p->field, w);
       o.scale = true;
-      o.scale_w = to_w_host(p->field, finv_host(p->field, (1ull << it.log_n) % field_modulus(p->field)));
+      o.scale_c = finv_host(p->field, (1ull << it.log_n) % field_modulus(p->field));
+      o.scale_w = to_w_host(p->field, o.scale_c);
     }
     const uint64_t n = 1ull << it.log_n;
     omr_status st = run_transform(ctx, p->field, (const char *)d_in + it.in_offset * es,

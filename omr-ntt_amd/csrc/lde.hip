@@ -104,7 +104,8 @@ extern "C" omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const voi
   if (!tuned_schedule(field, logM, s)) default_schedule(field, logM, s);
   XformOpts o;
   o.scale = true;
-  o.scale_w = to_w_host(field, finv_host(field, M % field_modulus(field)));
+  o.scale_c = finv_host(field, M % field_modulus(field));
+  o.scale_w = to_w_host(field, o.scale_c);
   if (sinv != 1) {
     o.post_lo = sh->lo;
     o.post_hi = sh->hi;
