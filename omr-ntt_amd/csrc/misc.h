@@ -137,13 +137,20 @@ __global__ void __launch_bounds__(256) k_fold(const typename F::T *__restrict__ 
   const uint64_t t1 = t0 + tpc < nt ? t0 + tpc : nt;
   const T *__restrict__ col = x + i + m * t0;
   T acc = 0;
-  if (t1 - t0 == kFoldRows) {  // full chunk: every load issued before the adds, four chains
+  const uint64_t cnt = t1 > t0 ? t1 - t0 : 0;
+  if (cnt == kFoldRows) {  // full chunk: every load issued before the adds, four chains
     T a[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int r = 0; r < kFoldRows; r++) a[r & 3] = F::add(a[r & 3], col[(uint64_t)r * m]);
     acc = F::add(F::add(a[0], a[1]), F::add(a[2], a[3]));
+  } else if (cnt < kFoldRows) {  // short chunk (a last level): the same, loads guarded
+    T a[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < kFoldRows; r++)
+      if ((uint64_t)r < cnt) a[r & 3] = F::add(a[r & 3], col[(uint64_t)r * m]);
+    acc = F::add(F::add(a[0], a[1]), F::add(a[2], a[3]));
   } else {
-    for (uint64_t t = 0; t < t1 - t0; t++) acc = F::add(acc, col[m * t]);
+    for (uint64_t t = 0; t < cnt; t++) acc = F::add(acc, col[m * t]);
   }
   part[(uint64_t)blockIdx.y * m + i] = acc;
 }
