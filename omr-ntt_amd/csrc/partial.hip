@@ -142,6 +142,7 @@ static omr_status prefix_fused(omr_ctx *ctx, int f, const void *d_in, uint64_t i
   const TwTables *tab;
   omr_status st = get_tables(ctx, f, logN, omega, &tab);
   if (st != OMR_OK) return st;
+  if (log_n1 == 0 || log_n1 > logN) return OMR_E_SCHEDULE;
   const std::vector<int> lr = prefix_split(log_n1);
   const int np = (int)lr.size();
   const uint32_t logS_last = logN - log_n1;
@@ -248,6 +249,8 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
                          m, m, o, stream);
   }
 
+  if (logN == 0)  // N = 1: the transform is the identity
+    return hipMemcpyAsync(d_out, d_in, es, hipMemcpyDeviceToDevice, stream) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
   if (method == 0 && (out_len & (out_len - 1)) == 0 &&
       tuned_partial(f, logN, ilog2u(out_len), method, log_n1)) {
     // autotuned choice
@@ -315,6 +318,10 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
 omr_status partial_batch_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
                               uint64_t batch, uint64_t omega, uint64_t out_len, hipStream_t stream) {
   const uint64_t N = 1ull << logN;
+  if (logN == 0)  // N = 1: every vector's transform is itself
+    return hipMemcpyAsync(d_out, d_in, batch * field_bytes(f), hipMemcpyDeviceToDevice, stream) == hipSuccess
+               ? OMR_OK
+               : OMR_E_DEVICE;
   if (out_len <= (1ull << kPrefixMaxLogR))
     return prefix_fused(ctx, f, d_in, N, d_out, out_len, logN, omega, out_len, prefix_log_n1(logN, out_len),
                         (uint32_t)batch, stream);

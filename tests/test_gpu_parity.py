@@ -94,6 +94,36 @@ def test_radix4_stockham8_and_constraints(fname):  # src/lib.rs:130-139
 
 
 @pytest.mark.parametrize("fname", FIELD_IDS)
+def test_smallest_domains(fname):
+    """N = 1 (omega = 1: identity; the reference accepts 1 as a power of two,
+    src/encoder.rs:24) and N = 2 (omega = p - 1), through every full and
+    partial encoder on the host path, the device forward and inverse."""
+    F = field(fname)
+    assert E.root_of_unity(F, 1) == 1 and E.root_of_unity(F, 2) == F.p - 1
+    for n in (1, 2):
+        x = inputs(F, n, 90 + n)
+        want = O.ntt(F.id, x)
+        a = [int(v) for v in x]
+        assert [int(v) for v in want] == (a if n == 1 else [(a[0] + a[1]) % F.p, (a[0] - a[1]) % F.p])
+        for enc in E.encoders.all(n.bit_length() - 1) + E.encoders.all_partial(1):
+            buf = x.astype(F.np_dtype)
+            enc.ntt(buf, E.NttDomain(n, F))
+            m = n if enc.s == 0 else min(2 * enc.s, n)
+            assert np.array_equal(buf[:m].astype(np.uint64), want[:m]), (enc.name(), n)
+        t = E.to_device(x, F)
+        E.ntt_device(t, t, F)
+        assert np.array_equal(E.from_device(t, F), want), n
+        E.intt_device(t, t, F)
+        assert np.array_equal(E.from_device(t, F), x), n
+        xb = inputs(F, 3 * n, 95 + n)  # three receivers' vectors, their whole prefix
+        out = E.empty_device(3 * n, F)
+        E.partial_batch_device(E.to_device(xb, F), out, F, n, 3, n)
+        got = E.from_device(out, F)
+        for b in range(3):
+            assert np.array_equal(got[b * n:(b + 1) * n], O.ntt(F.id, xb[b * n:(b + 1) * n])), (n, b)
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
 def test_dft_variant_agrees(fname):  # src/lib.rs:85-88
     F = field(fname)
     for n in (64, 128, 256, 4096):
