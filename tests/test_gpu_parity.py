@@ -615,3 +615,47 @@ def test_batched_config4_full_size():
         assert np.array_equal(ys[o:o + (1 << ln)], exp), idx
     plan.close()
     plan_inv.close()
+
+
+MAX_LOG = {"BabyBear": 27, "KoalaBear": 24, "TeddyBear": 30, "Goldilocks": 32}
+
+
+def _sum_mod_p(t, p, sign=False):
+    """sum_j (+-1)^j t[j] mod p, exact: 32-bit halves summed in int64 chunks."""
+    total = 0
+    chunk = 1 << 26
+    for s0 in range(0, t.numel(), chunk):
+        v = t[s0:s0 + chunk].to(torch.int64)
+        lo, hi = v & 0xFFFFFFFF, (v >> 32) & 0xFFFFFFFF
+        if sign:  # chunk starts at an even index
+            total += int(lo[0::2].sum()) + (int(hi[0::2].sum()) << 32)
+            total -= int(lo[1::2].sum()) + (int(hi[1::2].sum()) << 32)
+        else:
+            total += int(lo.sum()) + (int(hi.sum()) << 32)
+    return total % p
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_maximum_sizes(fname):
+    """Each field at its two-adicity limit (the largest N NttDomain::new
+    accepts, src/encoder.rs:28-32): forward then inverse gives the input
+    back, and X[0] = sum x, X[N/2] = sum (-1)^j x_j (closed forms, summed on
+    the device in exact integer chunks)."""
+    F = field(fname)
+    n = 1 << MAX_LOG[fname]
+    if F.bytes == 8:
+        x = torch.randint(0, 2**62, (n,), dtype=torch.int64, device="cuda")
+    else:  # int32 words < min(p, 2^31)
+        x = torch.randint(0, min(F.p, 2**31 - 1), (n,), dtype=torch.int32, device="cuda")
+    X = torch.empty_like(x)
+    E.ntt_device(x, X, F)
+    as_u = (lambda v: v.to(torch.int64) & 0xFFFFFFFF) if F.bytes == 4 else (lambda v: v)
+    got0 = int(as_u(X[0:1]).item()) % (1 << 64)
+    gotn2 = int(as_u(X[n // 2:n // 2 + 1]).item()) % (1 << 64)
+    assert got0 == _sum_mod_p(x, F.p), fname
+    assert gotn2 == _sum_mod_p(x, F.p, sign=True), fname
+    back = torch.empty_like(x)
+    E.intt_device(X, back, F)
+    assert torch.equal(back, x), fname
+    del x, X, back
+    torch.cuda.empty_cache()
