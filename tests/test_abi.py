@@ -65,6 +65,8 @@ def test_validation_before_device_work():
     gl = E.Goldilocks
     w = E.root_of_unity(gl, 1024)
     assert _call_ntt(2, 0, 1000, w) == 2          # not a power of two
+    assert _call_ntt(2, 0, 0, w) == 2             # empty: 0 is not a power of two (src/encoder.rs:24)
+    assert _call_ntt(3, 0, 1 << 31, 1) == 3       # beyond TeddyBear's two-adicity (30)
     assert _call_ntt(0, 0, 1 << 28, 1) == 3       # beyond BabyBear two-adicity
     assert _call_ntt(9, 0, 1024, w) == 4          # unknown field
     assert _call_ntt(2, 42, 1024, w) == 5         # unknown variant
