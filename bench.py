@@ -339,6 +339,54 @@ def summarize_profiles(prof_dir, out_path, logn=24):
     return res
 
 
+def summarize_sq(sq_dir, clk_dir, logn=24):
+    """Per-pass SQ counters of the forward transform (Goldilocks, U = 39
+    instantiation) from two rocprofv3 --pmc runs of this bench: the wait / LDS
+    counters, and GRBM_COUNT with kernel durations (--kernel-trace) for the
+    clock and the VALU utilisation (4 cycles per wave64 VALU instruction on a
+    16-lane SIMD, 1024 SIMDs).  Markdown on stdout."""
+    import glob
+    import sqlite3
+
+    def load(d, with_dur=False):
+        db = glob.glob(os.path.join(d, "**", "*results.db"), recursive=True)[0]
+        with sqlite3.connect(db) as c:
+            pm = c.execute("select dispatch_id, name, counter_name, counter_value from pmc_events").fetchall()
+            dur = dict(c.execute("select dispatch_id, duration from kernels").fetchall()) if with_dur else {}
+        per = {}
+        for did, name, cn, v in pm:
+            if "k_stockham_pass<omr::Goldilocks," in name and (", 39," in name or ", 39>" in name):
+                per.setdefault(did, {})
+                per[did][cn] = per[did].get(cn, 0.0) + float(v)
+        return per, dur
+
+    sys.path.insert(0, os.path.join(ROOT, "omr-ntt_amd"))
+    import omr_ntt as E
+    npass = len(E.default_schedule(E.Goldilocks, 1 << logn))
+    sq, _ = load(sq_dir)
+    ck, dur = load(clk_dir, True)
+
+    def by_pass(per):  # skip the first transform, average each pass index
+        ids = sorted(per)[npass:]
+        ids = ids[:len(ids) // npass * npass]
+        return [{k: sum(per[i][k] for i in ids[p::npass]) / len(ids[p::npass]) for k in per[ids[p]]}
+                for p in range(npass)], [ids[p::npass] for p in range(npass)]
+
+    sqp, _ = by_pass(sq)
+    ckp, ckids = by_pass(ck)
+    print("| pass | µs | clock (GHz) | VALU instr (wave) | VALU util | wait / wave-cycle | LDS active / wave-cycle | "
+          "bank-conflict / LDS cycle |")
+    print("|---|---|---|---|---|---|---|---|")
+    for p in range(npass):
+        us = sum(dur[i] for i in ckids[p]) / len(ckids[p]) / 1e3
+        ghz = ckp[p]["GRBM_COUNT"] / 8 / us / 1e3  # one GRBM instance per XCD
+        util = 4 * ckp[p]["SQ_INSTS_VALU"] / (1024 * us * 1e3 * ghz)
+        wc = sqp[p]["SQ_WAVE_CYCLES"]
+        print(f"| {p} | {us:.1f} | {ghz:.2f} | {ckp[p]['SQ_INSTS_VALU']:.3g} | {util:.2f} | "
+              f"{sqp[p]['SQ_WAIT_INST_ANY'] / wc:.2f} | {sqp[p]['SQ_ACTIVE_INST_LDS'] / wc:.3f} | "
+              f"{sqp[p]['SQ_LDS_BANK_CONFLICT'] / max(sqp[p]['SQ_ACTIVE_INST_LDS'], 1):.2f} |")
+
+
 def summarize_trace(trace_dir, logn=24):
     """Kernel-duration table of a rocprofv3 --kernel-trace run of this bench
     (calls, mean/min/max microseconds per kernel) and the forward
@@ -393,11 +441,16 @@ def main():
                     help="no GPU work: summarise the rocprofv3 --pmc runs under DIR (copy_fetch, copy_write, "
                          "fetch, write, valu) into per-pass HBM traffic and VALU work (see --profile-out)")
     ap.add_argument("--profile-out", default=os.path.join(ROOT, "profiles", "traffic_k_stockham_pass_goldilocks_2p24.json"))
+    ap.add_argument("--summarize-sq", nargs=2, metavar=("SQ_DIR", "CLOCK_DIR"),
+                    help="print per-pass SQ counters and clock from two rocprofv3 --pmc runs of this bench")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="no GPU work: print the kernel-duration table of a rocprofv3 --kernel-trace run under DIR")
     args = ap.parse_args()
     if args.summarize_profiles:
         summarize_profiles(args.summarize_profiles, args.profile_out, args.logn)
+        return
+    if args.summarize_sq:
+        summarize_sq(*args.summarize_sq, args.logn)
         return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace, args.logn)
