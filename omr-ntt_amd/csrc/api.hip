@@ -85,8 +85,19 @@ omr_status validate_domain(int f, uint64_t n, uint64_t omega) {
   if (n == 0 || (n & (n - 1))) return OMR_E_SIZE;
   if (ilog2(n) > kTwoAdicity[f]) return OMR_E_TWO_ADICITY;
   if (omega >= kP[f]) return OMR_E_OMEGA;
+  // the primitivity check costs two modular powers: the last few domains
+  // that passed it are remembered per thread
+  struct Seen {
+    int f;
+    uint64_t n, omega;
+  };
+  thread_local Seen seen[8] = {};
+  thread_local unsigned next = 0;
+  for (const Seen &d : seen)
+    if (d.n == n && d.omega == omega && d.f == f) return OMR_OK;
   if (fpow_host(f, omega, n) != 1) return OMR_E_OMEGA;
   if (n > 1 && fpow_host(f, omega, n / 2) == 1) return OMR_E_OMEGA;
+  seen[next++ % 8] = {f, n, omega};
   return OMR_OK;
 }
 

@@ -162,10 +162,18 @@ def context(device=0):
     return _ctx
 
 
+_roots = {}
+
+
 def root_of_unity(field, n):
-    w = ctypes.c_uint64()
-    _check(lib().omr_root_of_unity(field.id, n, ctypes.byref(w)), f"root_of_unity(N={n})")
-    return w.value
+    """F::get_root_of_unity(N) (the ark-ff convention g^((p-1)/N)), cached."""
+    key = (field.id, n)
+    w = _roots.get(key)
+    if w is None:
+        c = ctypes.c_uint64()
+        _check(lib().omr_root_of_unity(field.id, n, ctypes.byref(c)), f"root_of_unity(N={n})")
+        w = _roots[key] = c.value
+    return w
 
 
 def default_schedule(field, n):
