@@ -659,3 +659,120 @@ def test_maximum_sizes(fname):
     assert torch.equal(back, x), fname
     del x, X, back
     torch.cuda.empty_cache()
+
+
+def test_random_shapes_against_oracle():
+    """Seeded sweep over shapes the fixed tests do not enumerate: every field,
+    sizes 2^1..2^14, every full encoder where it applies, batches, prefix and
+    stride-subset lengths (ragged prefixes included), explicit methods and
+    sub-NTT sizes, explicit schedules and register radices, inverse."""
+    rng = np.random.default_rng(2024)
+    full = [E.ARK_RADIX2, E.DFT, E.LAMBDA_BOWERS, E.LAMBDA_RADIX4, E.PLONKY3_LAYER_SPLIT,
+            E.WINTERFELL_FOUR_STEP, E.TFHE_STOCKHAM_RADIX8]
+    for case in range(250):
+        F = E.FIELDS[int(rng.integers(0, 4))]
+        ln = int(rng.integers(1, 15))
+        n = 1 << ln
+        x = inputs(F, n, 5000 + case)
+        X = O.ntt(F.id, x)
+        t = E.to_device(x, F)
+        var = full[int(rng.integers(0, len(full)))]
+        if lib_supports(var, ln) and not (var == E.DFT and ln > 12):
+            out = E.empty_device(n, F)
+            E.ntt_device(t, out, F, var)
+            assert np.array_equal(E.from_device(out, F), X), (case, F.name, ln, var)
+        b = int(rng.integers(1, 5))
+        xb = inputs(F, n * b, 6000 + case)
+        tb = E.to_device(xb, F)
+        ob = E.empty_device(n * b, F)
+        E.ntt_device(tb, ob, F, batch=b, n=n)
+        gb = E.from_device(ob, F)
+        for i in range(b):
+            assert np.array_equal(gb[i * n:(i + 1) * n], O.ntt(F.id, xb[i * n:(i + 1) * n])), (case, i)
+        out_len = int(rng.integers(1, n + 1))
+        meth = int(rng.integers(0, 4))
+        l1 = int(rng.integers(0, ln + 1)) if meth == 2 else 0
+        po = E.empty_device(out_len, F)
+        try:
+            E.partial_device(t, po, F, out_len, mode=E.PREFIX, method=meth, log_n1=l1)
+        except E.OmrError as err:  # an explicit sub-NTT size that cannot serve this prefix
+            assert err.status == 11 and meth == 2, (case, ln, out_len, meth, l1)
+        else:
+            assert np.array_equal(E.from_device(po, F), X[:out_len]), (case, F.name, ln, out_len, meth, l1)
+        m = 1 << int(rng.integers(0, ln + 1))
+        so = E.empty_device(m, F)
+        E.partial_device(t, so, F, m, mode=E.STRIDE)
+        assert np.array_equal(E.from_device(so, F), X[::n // m]), (case, F.name, ln, m)
+        maxlr = 12 if F.bytes == 8 else 13
+        parts, left = [], ln
+        while left:
+            r = int(rng.integers(1, min(left, maxlr) + 1))
+            parts.append(r)
+            left -= r
+        if len(parts) <= 8:
+            le = int(rng.integers(0, 5))
+            so2 = E.empty_device(n, F)
+            E.ntt_schedule_device(t, so2, F, parts, log_e=le if le != 1 else 0)
+            assert np.array_equal(E.from_device(so2, F), X), (case, F.name, parts, le)
+            E.ntt_schedule_device(so2, so2, F, parts, log_e=le if le != 1 else 0, inverse=True)
+            assert torch.equal(so2, t), (case, F.name, parts, le)
+
+
+def lib_supports(variant, log_n):
+    return bool(E.lib().omr_variant_supports(variant, log_n))
+
+
+def test_random_structured_against_oracle():
+    """Seeded sweep over the larger and structured paths: prefixes of
+    N = 2^15..2^20 (one- and multi-pass fused, pruned full transform), the
+    batched prefix, coset LDE / decode with random blowup, shift and columns,
+    and uneven batch plans in every execution mode."""
+    rng = np.random.default_rng(77)
+    for case in range(24):
+        F = E.FIELDS[case % 4]
+        ln = int(rng.integers(15, 21))
+        n = 1 << ln
+        x = inputs(F, n, 7000 + case)
+        X = O.ntt(F.id, x)
+        t = E.to_device(x, F)
+        out_len = int(min(n, 2 ** rng.uniform(1, ln)))
+        po = E.empty_device(out_len, F)
+        E.partial_device(t, po, F, out_len, mode=E.PREFIX)
+        assert np.array_equal(E.from_device(po, F), X[:out_len]), (case, F.name, ln, out_len)
+        b, k = int(rng.integers(1, 4)), int(min(n, 2 ** rng.uniform(0, ln - 2)))
+        xb = inputs(F, n * b, 7100 + case)
+        ob = E.empty_device(k * b, F)
+        E.partial_batch_device(E.to_device(xb, F), ob, F, n, b, k)
+        gb = E.from_device(ob, F)
+        for i in range(b):
+            assert np.array_equal(gb[i * k:(i + 1) * k], O.ntt(F.id, xb[i * n:(i + 1) * n])[:k]), (case, i, k)
+        blow = 1 << int(rng.integers(1, 5))
+        lm = int(rng.integers(0, min(14, F.TWO_ADICITY - 4) + 1))
+        m, cols = 1 << lm, int(rng.integers(1, 4))
+        shift = int(rng.integers(2, 1000)) if case % 2 else 0
+        c = inputs(F, m * cols, 7200 + case)
+        ev = E.empty_device(m * blow * cols, F)
+        E.coset_lde_device(E.to_device(c, F), ev, F, m, cols, blow, shift)
+        got = E.from_device(ev, F)
+        for j in range(cols):
+            want = O.coset_lde(F.id, c[j * m:(j + 1) * m], blow, shift if shift else None)
+            assert np.array_equal(got[j * m * blow:(j + 1) * m * blow], want), (case, m, blow, shift, j)
+        back = E.empty_device(m * cols, F)
+        nz = torch.zeros(1, dtype=torch.int32, device="cuda")
+        E.coset_decode_device(ev, back, F, m, cols, blow, shift, nonzero=nz)
+        assert np.array_equal(E.from_device(back, F), c) and int(nz.item()) == 0, (case, m, blow)
+    for case in range(6):
+        F = E.FIELDS[case % 4]
+        items, total = _batch_items(rng, int(rng.integers(1, 40)), 4, 15, layout_gap=int(rng.integers(0, 5)))
+        x = inputs(F, total, 7300 + case)
+        t_in = E.to_device(x, F)
+        plan = E.BatchPlan(F, items)
+        for mode in ("persistent", "graph", "plain"):
+            out = torch.zeros_like(t_in)
+            plan.execute(t_in, out, mode)
+            got = E.from_device(out, F)
+            for (io, oo, ln, inv) in items:
+                seg = x[io:io + (1 << ln)]
+                want = O.intt(F.id, seg) if inv else O.ntt(F.id, seg)
+                assert np.array_equal(got[oo:oo + (1 << ln)], want), (case, mode, ln, inv)
+        plan.close()
