@@ -56,11 +56,20 @@ void par_memcpy(void *dst, const void *src, size_t bytes, int threads) {
     std::memcpy(dst, src, bytes);
     return;
   }
+  // no exception may leave the C ABI: a piece whose thread cannot be
+  // started is copied by the calling thread
   std::vector<std::thread> pool;
-  for (size_t o = piece; o < bytes; o += piece) {
-    const size_t l = std::min(piece, bytes - o);
-    pool.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, l); });
+  size_t o = piece;
+  try {
+    pool.reserve((bytes + piece - 1) / piece);
+    for (; o < bytes; o += piece) {
+      const size_t l = std::min(piece, bytes - o);
+      pool.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, l); });
+    }
+  } catch (...) {
   }
+  for (size_t r = o; r < bytes; r += piece)
+    std::memcpy((char *)dst + r, (const char *)src + r, std::min(piece, bytes - r));
   std::memcpy(dst, src, piece);  // the calling thread takes the first piece
   for (auto &t : pool) t.join();
 }
@@ -158,8 +167,13 @@ extern "C" omr_status omr_host_alloc(size_t bytes, void **out) {
     (void)hipGetLastError();
     return OMR_E_ALLOC;
   }
-  std::lock_guard<std::mutex> lk(g_pin_mu);
-  g_pinned[(uintptr_t)p] = bytes;
+  try {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pinned[(uintptr_t)p] = bytes;
+  } catch (...) {  // the registry could not grow
+    (void)hipHostFree(p);
+    return OMR_E_ALLOC;
+  }
   *out = p;
   return OMR_OK;
 }
