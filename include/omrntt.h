@@ -15,7 +15,8 @@
  *    domain exactly.  omr_root_of_unity gives the ark-ff convention
  *    g^((p-1)/N).
  *  - Every function validates all arguments before touching the GPU and
- *    returns an omr_status; nothing panics or aborts.
+ *    returns an omr_status; nothing panics or aborts, and no C++ exception
+ *    leaves the library (host allocation failure: OMR_E_ALLOC).
  *  - `_device` entry points take device pointers and a hipStream_t (passed as
  *    void*, NULL = default stream) and are asynchronous on that stream.
  *    Calls sharing one omr_ctx share its scratch buffers: issue them on one

@@ -3,6 +3,7 @@
 // Host code only; the kernels live in pass_<field>.hip (Stockham passes) and
 // misc_kernels.hip.  See include/omrntt.h for the contract.
 #include <hip/hip_runtime.h>
+#include <new>
 
 #include <dlfcn.h>
 
@@ -509,7 +510,7 @@ static void load_default_tuning() {
   });
 }
 
-extern "C" omr_status omr_ctx_create(int device, omr_ctx **out) {
+extern "C" omr_status omr_ctx_create(int device, omr_ctx **out) try {
   if (!out) return OMR_E_NULL;
   load_default_tuning();
   *out = nullptr;
@@ -524,9 +525,13 @@ extern "C" omr_status omr_ctx_create(int device, omr_ctx **out) {
   c->device = device;
   *out = c;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
+extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) try {
   if (!ctx) return OMR_E_NULL;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
@@ -546,31 +551,47 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) {
   free_stage(ctx->stage);
   delete ctx;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_ctx_synchronize(omr_ctx *ctx, void *stream) {
+extern "C" omr_status omr_ctx_synchronize(omr_ctx *ctx, void *stream) try {
   if (!ctx) return OMR_E_NULL;
   return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_field_info(int field, uint64_t *modulus, uint64_t *generator,
-                                     int *two_adicity) {
+                                     int *two_adicity) try {
   if (!field_ok(field)) return OMR_E_FIELD;
   if (modulus) *modulus = kP[field];
   if (generator) *generator = kG[field];
   if (two_adicity) *two_adicity = kTwoAdicity[field];
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" int omr_field_bytes(int field) { return field_ok(field) ? field_bytes(field) : 0; }
 
-extern "C" omr_status omr_root_of_unity(int field, uint64_t n, uint64_t *omega) {
+extern "C" omr_status omr_root_of_unity(int field, uint64_t n, uint64_t *omega) try {
   if (!omega) return OMR_E_NULL;
   if (!field_ok(field)) return OMR_E_FIELD;
   if (n == 0 || (n & (n - 1))) return OMR_E_SIZE;
   if (ilog2(n) > kTwoAdicity[field]) return OMR_E_TWO_ADICITY;
   *omega = fpow_host(field, kG[field], (kP[field] - 1) / n);
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 static const char *kVariantNames[OMR_VARIANT_COUNT] = {
@@ -591,7 +612,7 @@ extern "C" int omr_variant_supports(int variant, uint32_t log_n) {
   return variant >= 0 && variant < OMR_VARIANT_COUNT;
 }
 
-extern "C" omr_status omr_default_schedule(int field, uint64_t n, int *log_radix, int *npass) {
+extern "C" omr_status omr_default_schedule(int field, uint64_t n, int *log_radix, int *npass) try {
   if (!log_radix || !npass) return OMR_E_NULL;
   if (!field_ok(field)) return OMR_E_FIELD;
   if (n == 0 || (n & (n - 1))) return OMR_E_SIZE;
@@ -602,6 +623,10 @@ extern "C" omr_status omr_default_schedule(int field, uint64_t n, int *log_radix
   *npass = (int)s.logR.size();
   for (size_t i = 0; i < s.logR.size(); i++) log_radix[i] = s.logR[i];
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 static bool is_partial_variant(int v) {
@@ -653,7 +678,7 @@ omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void
 
 extern "C" omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const void *d_in,
                                      void *d_out, uint64_t n, uint64_t batch, uint64_t omega,
-                                     uint64_t s, void *stream) {
+                                     uint64_t s, void *stream) try {
   if (variant < 0 || variant >= OMR_VARIANT_COUNT) return OMR_E_VARIANT;
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
@@ -685,10 +710,14 @@ extern "C" omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const
     if (st != OMR_OK) return st;
   }
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in, void *d_out,
-                                      uint64_t n, uint64_t batch, uint64_t omega, void *stream) {
+                                      uint64_t n, uint64_t batch, uint64_t omega, void *stream) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
   if (!ctx || !d_in || !d_out) return OMR_E_NULL;
@@ -703,12 +732,16 @@ extern "C" omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in,
   o.scale_w = to_w_host(field, o.scale_c);
   return run_transform(ctx, field, d_in, d_out, logN, finv_host(field, omega), s, batch, n, n, o,
                        (hipStream_t)stream);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_ntt_schedule_device(omr_ctx *ctx, int field, const void *d_in,
                                               void *d_out, uint64_t n, uint64_t batch,
                                               uint64_t omega, int inverse, const int *log_radix,
-                                              int npass, int log_e, void *stream) {
+                                              int npass, int log_e, void *stream) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
   if (!log_radix) return OMR_E_NULL;
@@ -738,11 +771,15 @@ extern "C" omr_status omr_ntt_schedule_device(omr_ctx *ctx, int field, const voi
     o.scale_w = to_w_host(field, o.scale_c);
   }
   return run_transform(ctx, field, d_in, d_out, logN, root, s, batch, n, n, o, (hipStream_t)stream);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_in, void *d_out,
                                          uint64_t n, uint64_t omega, uint64_t out_len, int mode,
-                                         int method, uint32_t log_n1, void *stream) {
+                                         int method, uint32_t log_n1, void *stream) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
   if (out_len == 0 || out_len > n) return OMR_E_OUTLEN;
@@ -757,11 +794,15 @@ extern "C" omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
   return partial_impl(ctx, field, d_in, d_out, ilog2(n), omega, out_len, mode, method, log_n1,
                       (hipStream_t)stream);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_partial_batch_device(omr_ctx *ctx, int field, const void *d_in, void *d_out,
                                                uint64_t n, uint64_t batch, uint64_t omega, uint64_t out_len,
-                                               void *stream) {
+                                               void *stream) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
   if (out_len == 0 || out_len > n) return OMR_E_OUTLEN;
@@ -770,19 +811,27 @@ extern "C" omr_status omr_partial_batch_device(omr_ctx *ctx, int field, const vo
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
   return partial_batch_impl(ctx, field, d_in, d_out, ilog2(n), batch, omega, out_len, (hipStream_t)stream);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_field_op_device(int field, int op, const void *d_a, const void *d_b,
-                                          void *d_out, uint64_t n, void *stream) {
+                                          void *d_out, uint64_t n, void *stream) try {
   if (!d_a || !d_b || !d_out) return OMR_E_NULL;
   if (!field_ok(field)) return OMR_E_FIELD;
   if (op < 0 || op > 4) return OMR_E_ARG;
   return launch_field_op(field, op, d_a, d_b, d_out, n, (hipStream_t)stream);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // NttEncoder::ntt on a host buffer (src/encoder.rs:70-74): copy in, run, copy out.
 extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n,
-                              uint64_t omega, uint64_t s) {
+                              uint64_t omega, uint64_t s) try {
   if (variant < 0 || variant >= OMR_VARIANT_COUNT) return OMR_E_VARIANT;
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
@@ -814,6 +863,10 @@ extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, u
   }
   if (st != OMR_OK) return st;
   return device_to_host(ctx, buf, result, (size_t)copy_back * es);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // ---- host-buffer forms of the device entry points ---------------------------
@@ -826,7 +879,7 @@ static omr_status host_stage(omr_ctx *ctx, int i, size_t bytes, const void *src)
   return src ? host_to_device(ctx, ctx->hio[i].p, src, bytes) : OMR_OK;
 }
 
-extern "C" omr_status omr_intt(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t omega) {
+extern "C" omr_status omr_intt(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t omega) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
   if (!ctx || !buf) return OMR_E_NULL;
@@ -836,10 +889,14 @@ extern "C" omr_status omr_intt(omr_ctx *ctx, int field, void *buf, uint64_t n, u
   if ((st = host_stage(ctx, 0, bytes, buf)) != OMR_OK) return st;
   if ((st = omr_intt_device(ctx, field, ctx->hio[0].p, ctx->hio[0].p, n, 1, omega, nullptr)) != OMR_OK) return st;
   return device_to_host(ctx, buf, ctx->hio[0].p, bytes);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_ntt_batched(omr_ctx *ctx, int field, void *buf, uint64_t n, uint64_t batch,
-                                      uint64_t omega) {
+                                      uint64_t omega) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
   if (batch == 0) return OMR_E_ARG;
@@ -852,10 +909,14 @@ extern "C" omr_status omr_ntt_batched(omr_ctx *ctx, int field, void *buf, uint64
                            nullptr)) != OMR_OK)
     return st;
   return device_to_host(ctx, buf, ctx->hio[0].p, bytes);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_coset_lde(omr_ctx *ctx, int field, const void *coeffs, void *evals, uint64_t n,
-                                    uint64_t ncols, uint32_t blowup, uint64_t shift) {
+                                    uint64_t ncols, uint32_t blowup, uint64_t shift) try {
   uint64_t sh = shift;
   uint32_t logM;
   omr_status st = lde_args(field, n, ncols, blowup, sh, logM);
@@ -871,10 +932,14 @@ extern "C" omr_status omr_coset_lde(omr_ctx *ctx, int field, const void *coeffs,
       OMR_OK)
     return st;
   return device_to_host(ctx, evals, ctx->hio[1].p, out_bytes);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_coset_decode(omr_ctx *ctx, int field, const void *evals, void *coeffs, uint64_t n,
-                                       uint64_t ncols, uint32_t blowup, uint64_t shift, uint32_t *nonzero) {
+                                       uint64_t ncols, uint32_t blowup, uint64_t shift, uint32_t *nonzero) try {
   uint64_t sh = shift;
   uint32_t logM;
   omr_status st = lde_args(field, n, ncols, blowup, sh, logM);
@@ -892,4 +957,8 @@ extern "C" omr_status omr_coset_decode(omr_ctx *ctx, int field, const void *eval
     return st;
   if ((st = device_to_host(ctx, coeffs, ctx->hio[1].p, out_bytes)) != OMR_OK) return st;
   return nonzero ? device_to_host(ctx, nonzero, ctx->hio[2].p, sizeof(uint32_t)) : OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }

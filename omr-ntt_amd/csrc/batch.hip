@@ -9,6 +9,7 @@
 // are captured once into a hipGraph and replayed.  The plain form (one
 // launch sequence per vector) is kept for comparison.
 #include <algorithm>
+#include <new>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -84,13 +85,17 @@ static void batch_schedule(uint32_t logn, std::vector<int> &out) {
 
 }  // namespace omr
 
-extern "C" omr_status omr_batch_set_max_log_radix(int log_radix) {
+extern "C" omr_status omr_batch_set_max_log_radix(int log_radix) try {
   if (log_radix < 4 || log_radix > 12) return OMR_E_SCHEDULE;
   g_batch_max_log_radix = log_radix;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_batch_plan_destroy(omr_batch_plan *p) {
+extern "C" omr_status omr_batch_plan_destroy(omr_batch_plan *p) try {
   if (!p) return OMR_E_NULL;
   (void)hipDeviceSynchronize();
   if (p->ctx && !p->pinned.empty()) {
@@ -107,10 +112,14 @@ extern "C" omr_status omr_batch_plan_destroy(omr_batch_plan *p) {
   free_buf(p->scratch[1]);
   delete p;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_batch_item *items,
-                                            uint32_t count, omr_batch_plan **out) {
+                                            uint32_t count, omr_batch_plan **out) try {
   if (!field_ok(field)) return OMR_E_FIELD;
   if (!items || !out) return OMR_E_NULL;
   if (count == 0) return OMR_E_ARG;
@@ -304,10 +313,14 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
   }
   *out = p;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_batch_plan_info(const omr_batch_plan *p, uint64_t *total_elems,
-                                          uint32_t *nlaunch, uint32_t *ntasks) {
+                                          uint32_t *nlaunch, uint32_t *ntasks) try {
   if (!p) return OMR_E_NULL;
   if (total_elems) *total_elems = p->total_elems;
   if (nlaunch) *nlaunch = p->nlaunch;
@@ -317,6 +330,10 @@ extern "C" omr_status omr_batch_plan_info(const omr_batch_plan *p, uint64_t *tot
     *ntasks = s;
   }
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 static omr_status batch_launches(omr_batch_plan *p, const void *d_in, void *d_out, hipStream_t s) {
@@ -334,14 +351,18 @@ static omr_status batch_launches(omr_batch_plan *p, const void *d_in, void *d_ou
   return OMR_OK;
 }
 
-extern "C" omr_status omr_batch_execute(omr_batch_plan *p, const void *d_in, void *d_out, void *stream) {
+extern "C" omr_status omr_batch_execute(omr_batch_plan *p, const void *d_in, void *d_out, void *stream) try {
   if (!p || !d_in || !d_out) return OMR_E_NULL;
   if (hipSetDevice(p->ctx->device) != hipSuccess) return OMR_E_DEVICE;
   return batch_launches(p, d_in, d_out, (hipStream_t)stream);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_batch_execute_graph(omr_batch_plan *p, const void *d_in, void *d_out,
-                                              void *stream) {
+                                              void *stream) try {
   if (!p || !d_in || !d_out) return OMR_E_NULL;
   if (hipSetDevice(p->ctx->device) != hipSuccess) return OMR_E_DEVICE;
   if (!p->exec || p->g_in != d_in || p->g_out != d_out) {
@@ -369,12 +390,16 @@ extern "C" omr_status omr_batch_execute_graph(omr_batch_plan *p, const void *d_i
     p->g_out = d_out;
   }
   return hipGraphLaunch(p->exec, (hipStream_t)stream) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // Baseline form: one launch sequence per vector (the engine's plain
 // single-vector transform, one call per vector).
 extern "C" omr_status omr_batch_execute_plain(omr_batch_plan *p, const void *d_in, void *d_out,
-                                              void *stream) {
+                                              void *stream) try {
   if (!p || !d_in || !d_out) return OMR_E_NULL;
   omr_ctx *ctx = p->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -399,4 +424,8 @@ extern "C" omr_status omr_batch_execute_plain(omr_batch_plan *p, const void *d_i
     if (st != OMR_OK) return st;
   }
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }

@@ -7,6 +7,7 @@
 // hipMemcpy.  Copies below kStageMin bytes also take plain hipMemcpy, and
 // so does every copy when OMR_HOST_STAGING=0.
 #include <algorithm>
+#include <new>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -158,7 +159,7 @@ void free_stage(HostStage &s) {
 }  // namespace omr
 
 // Pinned host memory for callers that keep their buffers across calls.
-extern "C" omr_status omr_host_alloc(size_t bytes, void **out) {
+extern "C" omr_status omr_host_alloc(size_t bytes, void **out) try {
   if (!out) return OMR_E_NULL;
   *out = nullptr;
   if (bytes == 0) return OMR_E_ARG;
@@ -176,9 +177,13 @@ extern "C" omr_status omr_host_alloc(size_t bytes, void **out) {
   }
   *out = p;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_host_free(void *p) {
+extern "C" omr_status omr_host_free(void *p) try {
   if (!p) return OMR_E_NULL;
   {
     std::lock_guard<std::mutex> lk(g_pin_mu);
@@ -187,4 +192,8 @@ extern "C" omr_status omr_host_free(void *p) {
     g_pinned.erase(it);
   }
   return hipHostFree(p) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }

@@ -9,6 +9,7 @@
 //           pass fuses h^-j m^-1, stores j < n and counts nonzero j >= n
 //           (a codeword has none).
 #include <algorithm>
+#include <new>
 
 #include "dispatch.h"
 #include "internal.h"
@@ -53,7 +54,7 @@ using namespace omr;
 
 extern "C" omr_status omr_coset_lde_device(omr_ctx *ctx, int field, const void *d_coeffs,
                                            void *d_evals, uint64_t n, uint64_t ncols,
-                                           uint32_t blowup, uint64_t shift, void *stream) {
+                                           uint32_t blowup, uint64_t shift, void *stream) try {
   uint32_t logM;
   omr_status st = lde_args(field, n, ncols, blowup, shift, logM);
   if (st != OMR_OK) return st;
@@ -77,12 +78,16 @@ extern "C" omr_status omr_coset_lde_device(omr_ctx *ctx, int field, const void *
   }
   return run_columns(ctx, field, d_coeffs, d_evals, logM, w, s, ncols, n, 1ull << logM, o,
                      (hipStream_t)stream);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const void *d_evals,
                                               void *d_coeffs, uint64_t n, uint64_t ncols,
                                               uint32_t blowup, uint64_t shift, uint32_t *d_nonzero,
-                                              void *stream) {
+                                              void *stream) try {
   uint32_t logM;
   omr_status st = lde_args(field, n, ncols, blowup, shift, logM);
   if (st != OMR_OK) return st;
@@ -113,4 +118,8 @@ extern "C" omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const voi
   o.store_limit = n;
   o.nonzero_count = d_nonzero;
   return run_columns(ctx, field, d_evals, d_coeffs, logM, winv, s, ncols, M, n, o, hs);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }

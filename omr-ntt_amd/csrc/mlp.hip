@@ -8,6 +8,7 @@
 // Gradients are computed per sample into rows of a buffer and reduced in a
 // fixed order, so training is deterministic.
 #include <hip/hip_runtime.h>
+#include <new>
 
 #include <cmath>
 #include <cstdio>
@@ -171,7 +172,7 @@ extern "C" int omr_cost_model_num_params(void) { return kParams; }
 extern "C" int omr_cost_model_num_inputs(void) { return kIn; }
 extern "C" int omr_cost_model_num_outputs(void) { return kOut; }
 
-extern "C" omr_status omr_cost_model_destroy(omr_cost_model *cm) {
+extern "C" omr_status omr_cost_model_destroy(omr_cost_model *cm) try {
   if (!cm) return OMR_E_NULL;
   (void)hipDeviceSynchronize();
   mlp_free_work(cm);
@@ -179,25 +180,37 @@ extern "C" omr_status omr_cost_model_destroy(omr_cost_model *cm) {
     if (p) (void)hipFree(p);
   delete cm;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_cost_model_set_params(omr_cost_model *cm, const float *params) {
+extern "C" omr_status omr_cost_model_set_params(omr_cost_model *cm, const float *params) try {
   if (!cm || !params) return OMR_E_NULL;
   if (hipMemcpy(cm->P, params, kParams * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
   (void)hipMemset(cm->m, 0, kParams * sizeof(float));
   (void)hipMemset(cm->v, 0, kParams * sizeof(float));
   cm->adam_t = 0;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_cost_model_get_params(omr_cost_model *cm, float *params) {
+extern "C" omr_status omr_cost_model_get_params(omr_cost_model *cm, float *params) try {
   if (!cm || !params) return OMR_E_NULL;
   return hipMemcpy(params, cm->P, kParams * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess ? OMR_OK
                                                                                                 : OMR_E_DEVICE;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // He-uniform initialisation from a seeded host generator.
-extern "C" omr_status omr_cost_model_create(omr_ctx *ctx, uint32_t seed, omr_cost_model **out) {
+extern "C" omr_status omr_cost_model_create(omr_ctx *ctx, uint32_t seed, omr_cost_model **out) try {
   if (!ctx || !out) return OMR_E_NULL;
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
   omr_cost_model *cm = new (std::nothrow) omr_cost_model();
@@ -226,10 +239,14 @@ extern "C" omr_status omr_cost_model_create(omr_ctx *ctx, uint32_t seed, omr_cos
   }
   *out = cm;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 extern "C" omr_status omr_cost_model_features(int field, uint64_t n, uint64_t out_len, uint64_t batch,
-                                              float *x) {
+                                              float *x) try {
   if (!x) return OMR_E_NULL;
   if (!field_ok(field)) return OMR_E_FIELD;
   if (n == 0 || out_len == 0 || batch == 0) return OMR_E_SIZE;
@@ -238,6 +255,10 @@ extern "C" omr_status omr_cost_model_features(int field, uint64_t n, uint64_t ou
   x[2] = (float)std::log2((double)out_len) / 32.f;
   x[3] = (float)std::log2((double)batch) / 16.f;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 static omr_status mlp_upload_forward(omr_cost_model *cm, const float *x, uint32_t B) {
@@ -248,7 +269,7 @@ static omr_status mlp_upload_forward(omr_cost_model *cm, const float *x, uint32_
   return hipGetLastError() == hipSuccess ? OMR_OK : OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_cost_model_forward(omr_cost_model *cm, const float *x, uint32_t B, float *y) {
+extern "C" omr_status omr_cost_model_forward(omr_cost_model *cm, const float *x, uint32_t B, float *y) try {
   if (!cm || !x || !y) return OMR_E_NULL;
   if (B == 0) return OMR_E_ARG;
   if (hipSetDevice(cm->ctx->device) != hipSuccess) return OMR_E_DEVICE;
@@ -257,6 +278,10 @@ extern "C" omr_status omr_cost_model_forward(omr_cost_model *cm, const float *x,
   return hipMemcpy(y, cm->Y, (size_t)B * kOut * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess
              ? OMR_OK
              : OMR_E_DEVICE;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // One forward + backward over the (already uploaded) batch; leaves the summed
@@ -287,7 +312,7 @@ static omr_status mlp_upload_train(omr_cost_model *cm, const float *x, const flo
 }
 
 extern "C" omr_status omr_cost_model_gradients(omr_cost_model *cm, const float *x, const float *t,
-                                               const float *mask, uint32_t B, float *grad, float *loss) {
+                                               const float *mask, uint32_t B, float *grad, float *loss) try {
   if (!cm || !x || !t || !mask || !grad) return OMR_E_NULL;
   if (B == 0) return OMR_E_ARG;
   if (hipSetDevice(cm->ctx->device) != hipSuccess) return OMR_E_DEVICE;
@@ -298,12 +323,16 @@ extern "C" omr_status omr_cost_model_gradients(omr_cost_model *cm, const float *
   if (hipMemcpy(grad, cm->g, kParams * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return OMR_E_DEVICE;
   if (loss && hipMemcpy(loss, cm->Lsum, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return OMR_E_DEVICE;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // Full-batch Adam for `steps` steps; *loss_out = loss before the last step.
 extern "C" omr_status omr_cost_model_train(omr_cost_model *cm, const float *x, const float *t,
                                            const float *mask, uint32_t B, uint32_t steps, float lr,
-                                           float *loss_out) {
+                                           float *loss_out) try {
   if (!cm || !x || !t || !mask) return OMR_E_NULL;
   if (B == 0 || !(lr > 0.f)) return OMR_E_ARG;
   if (hipSetDevice(cm->ctx->device) != hipSuccess) return OMR_E_DEVICE;
@@ -324,11 +353,15 @@ extern "C" omr_status omr_cost_model_train(omr_cost_model *cm, const float *x, c
   if (hipDeviceSynchronize() != hipSuccess) return OMR_E_DEVICE;
   cm->trained = true;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // Schedule class the model scores lowest among the classes valid for the shape.
 extern "C" omr_status omr_cost_model_choose(omr_cost_model *cm, int field, uint64_t n, uint64_t out_len,
-                                            uint64_t batch, int *cls) {
+                                            uint64_t batch, int *cls) try {
   if (!cm || !cls) return OMR_E_NULL;
   float x[kIn], y[kOut];
   omr_status st = omr_cost_model_features(field, n, out_len, batch, x);
@@ -346,6 +379,10 @@ extern "C" omr_status omr_cost_model_choose(omr_cost_model *cm, int field, uint6
   if (best < 0) return OMR_E_SCHEDULE;
   *cls = best;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // Picks and installs a schedule for a shape.  With a trained model:
@@ -356,7 +393,7 @@ extern "C" omr_status omr_cost_model_choose(omr_cost_model *cm, int field, uint6
 // of classes timed.
 extern "C" omr_status omr_autotune_shape(omr_ctx *ctx, omr_cost_model *cm, int use_model, int field,
                                          uint64_t n, uint64_t out_len, uint64_t batch, int reps, int *cls,
-                                         int *timed) {
+                                         int *timed) try {
   if (!ctx || !cls) return OMR_E_NULL;
   omr_status st;
   int chosen = -1, ntimed = 0;
@@ -401,9 +438,13 @@ extern "C" omr_status omr_autotune_shape(omr_ctx *ctx, omr_cost_model *cm, int u
   *cls = chosen;
   if (timed) *timed = ntimed;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_cost_model_save(omr_cost_model *cm, const char *path) {
+extern "C" omr_status omr_cost_model_save(omr_cost_model *cm, const char *path) try {
   if (!cm || !path) return OMR_E_NULL;
   std::vector<float> h(kParams);
   omr_status st = omr_cost_model_get_params(cm, h.data());
@@ -414,9 +455,13 @@ extern "C" omr_status omr_cost_model_save(omr_cost_model *cm, const char *path) 
   bool ok = fwrite(hdr, sizeof(hdr), 1, fp) == 1 && fwrite(h.data(), sizeof(float), kParams, fp) == (size_t)kParams;
   fclose(fp);
   return ok ? OMR_OK : OMR_E_ARG;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_cost_model_load(omr_cost_model *cm, const char *path) {
+extern "C" omr_status omr_cost_model_load(omr_cost_model *cm, const char *path) try {
   if (!cm || !path) return OMR_E_NULL;
   FILE *fp = fopen(path, "rb");
   if (!fp) return OMR_E_ARG;
@@ -429,4 +474,8 @@ extern "C" omr_status omr_cost_model_load(omr_cost_model *cm, const char *path) 
   omr_status st = omr_cost_model_set_params(cm, h.data());
   if (st == OMR_OK) cm->trained = hdr[2] != 0;
   return st;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }

@@ -17,6 +17,7 @@
 // The timing autotuner measures every valid class of a shape; its best is
 // the reference the cost model's regret is measured against.
 #include <hip/hip_runtime.h>
+#include <new>
 
 #include <algorithm>
 #include <cmath>
@@ -138,7 +139,7 @@ extern "C" int omr_num_schedule_classes(void) { return kNumClasses; }
 
 extern "C" omr_status omr_schedule_class(int field, uint64_t n, uint64_t out_len, int cls,
                                          int *log_radix, int *npass, int *log_e, int *method,
-                                         uint32_t *log_n1) {
+                                         uint32_t *log_n1) try {
   if (!field_ok(field)) return OMR_E_FIELD;
   if (n == 0 || (n & (n - 1)) || out_len == 0 || out_len > n || (out_len & (out_len - 1)))
     return OMR_E_SIZE;
@@ -161,14 +162,22 @@ extern "C" omr_status omr_schedule_class(int field, uint64_t n, uint64_t out_len
   if (log_n1) *log_n1 = l1;
   if (npass) *npass = 0;
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // Times every valid class of a shape: times_us[cls] = median microseconds,
 // or -1 for classes invalid for the shape.  Inputs are pseudo-random field
 // elements in a context-owned buffer.
 extern "C" omr_status omr_tune_classes(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len,
-                                       uint64_t batch, int reps, float *times_us) {
+                                       uint64_t batch, int reps, float *times_us) try {
   return tune_classes_subset(ctx, field, n, out_len, batch, reps, nullptr, times_us);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // Times the valid classes with only[cls] set (every valid class when only is
@@ -293,7 +302,7 @@ extern "C" int omr_default_class(int field, uint64_t n, uint64_t out_len) {
 }
 
 // Installs class `cls` as the tuned schedule of (field, n, out_len).
-extern "C" omr_status omr_tune_set_class(int field, uint64_t n, uint64_t out_len, int cls) {
+extern "C" omr_status omr_tune_set_class(int field, uint64_t n, uint64_t out_len, int cls) try {
   if (!field_ok(field)) return OMR_E_FIELD;
   if (n == 0 || (n & (n - 1)) || out_len == 0 || out_len > n || (out_len & (out_len - 1)))
     return OMR_E_SIZE;
@@ -310,18 +319,26 @@ extern "C" omr_status omr_tune_set_class(int field, uint64_t n, uint64_t out_len
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned_partial[std::make_tuple(field, logN, logOut)] = {m, l1};
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_tune_clear(void) {
+extern "C" omr_status omr_tune_clear(void) try {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned.clear();
   g_tuned_partial.clear();
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
 // Text checkpoint of the tuned tables: one line per entry,
 //   F <field> <logN> <logE> <npass> <logR...>   or   P <field> <logN> <logOut> <method> <log_n1>
-extern "C" omr_status omr_tune_save(const char *path) {
+extern "C" omr_status omr_tune_save(const char *path) try {
   if (!path) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   FILE *fp = fopen(path, "w");
@@ -336,9 +353,13 @@ extern "C" omr_status omr_tune_save(const char *path) {
             kv.second.first, kv.second.second);
   fclose(fp);
   return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
 
-extern "C" omr_status omr_tune_load(const char *path) {
+extern "C" omr_status omr_tune_load(const char *path) try {
   if (!path) return OMR_E_NULL;
   FILE *fp = fopen(path, "r");
   if (!fp) return OMR_E_ARG;
@@ -386,4 +407,8 @@ extern "C" omr_status omr_tune_load(const char *path) {
   }
   fclose(fp);
   return st;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
 }
