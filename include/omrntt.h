@@ -198,6 +198,14 @@ omr_status omr_batch_execute_graph(omr_batch_plan *plan, const void *d_in, void 
 /* Baseline form: the single-vector transform launched once per vector. */
 omr_status omr_batch_execute_plain(omr_batch_plan *plan, const void *d_in, void *d_out,
                                    void *stream);
+/* Grouped form: the vectors of each (size, direction) as one batched
+ * transform of the pass engine under that size's tuned schedule; the first
+ * pass reads and the last pass writes through per-vector offset tables.
+ * The _graph variant captures the launches for a (d_in, d_out) pair and
+ * replays them; its capturing call runs them eagerly as well. */
+omr_status omr_batch_execute_grouped(omr_batch_plan *plan, const void *d_in, void *d_out, void *stream);
+omr_status omr_batch_execute_grouped_graph(omr_batch_plan *plan, const void *d_in, void *d_out,
+                                           void *stream);
 /* Largest pass radix (2^4 .. 2^12) the batch planner gives multi-pass vectors
  * of later plans (default 2^9). */
 omr_status omr_batch_set_max_log_radix(int log_radix);

@@ -405,22 +405,26 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
   X[0] = d_in;
   X[np] = d_out;
   const size_t vec_bytes = (size_t)N * es * batch;
-  // d_out doubles as a ping-pong buffer only when it holds whole vectors
-  const bool use_out = (o.store_limit >= N) && ((d_in != d_out) || (np % 2 == 0));
-  if (np >= 2) {
-    if ((st = ensure_buf(ctx->scratch[0], vec_bytes)) != OMR_OK) return st;
-    if (!use_out && np >= 3)
-      if ((st = ensure_buf(ctx->scratch[1], vec_bytes)) != OMR_OK) return st;
-  }
+  // d_out doubles as a ping-pong buffer only when it holds whole dense vectors
+  const bool use_out = !o.off_in && !o.off_out && (o.store_limit >= N) && ((d_in != d_out) || (np % 2 == 0));
+  void *S[2] = {o.scratch[0], o.scratch[1]};  // caller-owned, else the context's
+  auto need = [&](int k) -> omr_status {
+    if (S[k]) return OMR_OK;
+    const omr_status s2 = ensure_buf(ctx->scratch[k], vec_bytes);
+    if (s2 == OMR_OK) S[k] = ctx->scratch[k].p;
+    return s2;
+  };
+  if (np >= 2 && (st = need(0)) != OMR_OK) return st;
+  if (np >= 3 && !use_out && (st = need(1)) != OMR_OK) return st;
   for (int i = 1; i < np; i++) {
-    if (use_out) X[i] = ((np - i) % 2 == 0) ? d_out : ctx->scratch[0].p;
-    else X[i] = ((np - i) % 2 == 1) ? ctx->scratch[0].p : ctx->scratch[1].p;
+    if (use_out) X[i] = ((np - i) % 2 == 0) ? d_out : S[0];
+    else X[i] = ((np - i) % 2 == 1) ? S[0] : S[1];
   }
   // with use_out and an out-of-place call whose out vectors are strided, the
   // intermediates must be dense N-element vectors
   if ((use_out && np >= 2 && out_stride != N) || (!use_out && np >= 2)) {
-    if ((st = ensure_buf(ctx->scratch[1], vec_bytes)) != OMR_OK) return st;
-    for (int i = 1; i < np; i++) X[i] = ((np - i) % 2 == 1) ? ctx->scratch[0].p : ctx->scratch[1].p;
+    if ((st = need(1)) != OMR_OK) return st;
+    for (int i = 1; i < np; i++) X[i] = ((np - i) % 2 == 1) ? S[0] : S[1];
   }
 
   int consumed = 0;
@@ -447,6 +451,8 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.logC = logC;
     a.vec_stride_in = (i == 0) ? in_stride : N;
     a.vec_stride_out = (i == np - 1) ? out_stride : N;
+    a.vec_off_in = (i == 0) ? o.off_in : nullptr;
+    a.vec_off_out = (i == np - 1) ? o.off_out : nullptr;
     a.store_limit = (i == np - 1) ? o.store_limit : ~0ull;
     a.flags = 0;
     if (i > 0) a.flags |= kTwiddle;

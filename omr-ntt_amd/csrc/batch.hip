@@ -41,6 +41,19 @@ struct omr_batch_plan {
   hipGraphExec_t exec = nullptr;
   const void *g_in = nullptr;
   void *g_out = nullptr;
+  // grouped form: the vectors of one (size, direction), at most 65535 a
+  // group, run through the pass engine with per-vector offset tables
+  struct Group {
+    uint32_t log_n, inverse, count;
+    uint64_t root;         // the transform's root (inverse roots for inverse groups)
+    uint64_t *d_off;       // count input offsets, then count output offsets (elements)
+  };
+  std::vector<Group> groups;
+  omr::DevBuf gscratch[2];  // dense intermediates of the largest multi-pass group
+  hipGraph_t ggraph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void *gg_in = nullptr;
+  void *gg_out = nullptr;
 };
 
 namespace omr {
@@ -104,7 +117,12 @@ extern "C" omr_status omr_batch_plan_destroy(omr_batch_plan *p) try {
   }
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->ggraph) (void)hipGraphDestroy(p->ggraph);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  for (auto &g : p->groups) (void)hipFree(g.d_off);
+  free_buf(p->gscratch[0]);
+  free_buf(p->gscratch[1]);
   for (void *q : p->d_tasks) (void)hipFree(q);
   for (void *q : p->d_vecs) (void)hipFree(q);
   if (p->d_tsets) (void)hipFree(p->d_tsets);
@@ -304,6 +322,43 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
     p->grid.push_back((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tasks.size(), (uint64_t)cus * occ)));
   }
   p->nlaunch = (uint32_t)p->ntasks.size();
+
+  // grouped form: one group per (size, direction), chunks of 65535 vectors
+  if (st == OMR_OK) {
+    std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> by_shape;
+    for (uint32_t i = 0; i < count; i++) by_shape[{items[i].log_n, items[i].inverse}].push_back(i);
+    uint64_t gmax = 0;
+    for (auto it = by_shape.rbegin(); it != by_shape.rend() && st == OMR_OK; ++it) {  // largest first
+      const std::vector<uint32_t> &idx = it->second;
+      const uint32_t logn = it->first.first, inv = it->first.second;
+      uint64_t w;
+      omr_root_of_unity(field, 1ull << logn, &w);
+      if (inv) w = finv_host(field, w);
+      Schedule sch;
+      if (!tuned_schedule(field, logn, sch)) default_schedule(field, logn, sch);
+      for (size_t c0 = 0; c0 < idx.size() && st == OMR_OK; c0 += 65535) {
+        const uint32_t cnt = (uint32_t)std::min<size_t>(65535, idx.size() - c0);
+        std::vector<uint64_t> off(2 * (size_t)cnt);
+        for (uint32_t j = 0; j < cnt; j++) {
+          off[j] = items[idx[c0 + j]].in_offset;
+          off[cnt + j] = items[idx[c0 + j]].out_offset;
+        }
+        omr_batch_plan::Group g{logn, inv, cnt, w, nullptr};
+        if (hipMalloc((void **)&g.d_off, off.size() * sizeof(uint64_t)) != hipSuccess) {
+          (void)hipGetLastError();
+          st = OMR_E_ALLOC;
+          break;
+        }
+        p->groups.push_back(g);
+        if (hipMemcpy(g.d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice) != hipSuccess)
+          st = OMR_E_DEVICE;
+        if (sch.logR.size() >= 2) gmax = std::max<uint64_t>(gmax, (uint64_t)cnt << logn);
+      }
+    }
+    if (st == OMR_OK && gmax) {
+      if ((st = ensure_buf(p->gscratch[0], gmax * es)) == OMR_OK) st = ensure_buf(p->gscratch[1], gmax * es);
+    }
+  }
   if (st != OMR_OK) {
     // the context lock is held here: unpin directly, then free the rest
     for (auto &k : p->pinned) pin_tables(ctx, std::get<0>(k), std::get<1>(k), std::get<2>(k), -1);
@@ -423,6 +478,78 @@ extern "C" omr_status omr_batch_execute_plain(omr_batch_plan *p, const void *d_i
                                   (hipStream_t)stream);
     if (st != OMR_OK) return st;
   }
+  return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
+}
+
+// Grouped form: each group's vectors as one batched transform of the pass
+// engine (the tuned schedule of their size), the first pass reading and the
+// last pass writing through the group's offset tables, the intermediates
+// dense in the plan's own buffers.
+static omr_status grouped_launches(omr_batch_plan *p, const void *d_in, void *d_out, hipStream_t s) {
+  for (const auto &g : p->groups) {
+    Schedule sch;
+    if (!tuned_schedule(p->field, g.log_n, sch)) default_schedule(p->field, g.log_n, sch);
+    XformOpts o;
+    o.off_in = g.d_off;
+    o.off_out = g.d_off + g.count;
+    if ((p->gscratch[0].bytes >> g.log_n) / field_bytes(p->field) >= g.count) {
+      o.scratch[0] = p->gscratch[0].p;
+      o.scratch[1] = p->gscratch[1].p;
+    }
+    if (g.inverse) {
+      o.scale = true;
+      o.scale_c = finv_host(p->field, (1ull << g.log_n) % field_modulus(p->field));
+      o.scale_w = to_w_host(p->field, o.scale_c);
+    }
+    const omr_status st = run_transform(p->ctx, p->field, d_in, d_out, g.log_n, g.root, sch, g.count, 0, 0, o, s);
+    if (st != OMR_OK) return st;
+  }
+  return OMR_OK;
+}
+
+extern "C" omr_status omr_batch_execute_grouped(omr_batch_plan *p, const void *d_in, void *d_out,
+                                                void *stream) try {
+  if (!p || !d_in || !d_out) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(p->ctx->mu);
+  if (hipSetDevice(p->ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  return grouped_launches(p, d_in, d_out, (hipStream_t)stream);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
+}
+
+// The grouped launches captured once per (d_in, d_out) and replayed.  The
+// call that captures first runs the launches eagerly, which creates any
+// table or buffer they need outside the capture.
+extern "C" omr_status omr_batch_execute_grouped_graph(omr_batch_plan *p, const void *d_in, void *d_out,
+                                                      void *stream) try {
+  if (!p || !d_in || !d_out) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(p->ctx->mu);
+  if (hipSetDevice(p->ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  if (p->gexec && p->gg_in == d_in && p->gg_out == d_out)
+    return hipGraphLaunch(p->gexec, (hipStream_t)stream) == hipSuccess ? OMR_OK : OMR_E_DEVICE;
+  omr_status st = grouped_launches(p, d_in, d_out, (hipStream_t)stream);
+  if (st != OMR_OK) return st;
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec), p->gexec = nullptr;
+  if (p->ggraph) (void)hipGraphDestroy(p->ggraph), p->ggraph = nullptr;
+  if (!p->cap_stream && hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking) != hipSuccess)
+    return OMR_E_DEVICE;
+  if (hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return OMR_E_DEVICE;
+  st = grouped_launches(p, d_in, d_out, p->cap_stream);
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(p->cap_stream, &g) != hipSuccess || st != OMR_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st != OMR_OK ? st : OMR_E_DEVICE;
+  }
+  p->ggraph = g;
+  if (hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0) != hipSuccess) return OMR_E_DEVICE;
+  p->gg_in = d_in;
+  p->gg_out = d_out;
   return OMR_OK;
 } catch (const std::bad_alloc &) {
   return OMR_E_ALLOC;

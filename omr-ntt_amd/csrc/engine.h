@@ -32,6 +32,8 @@ struct PassArgs {
   uint32_t *nonzero_count;  // if set: outputs at index >= store_limit that are nonzero are counted
   uint64_t vec_stride_in;   // elements between consecutive input vectors
   uint64_t vec_stride_out;  // elements between consecutive output vectors
+  const uint64_t *vec_off_in;   // if set: vector v's input starts at element vec_off_in[v]
+  const uint64_t *vec_off_out;  // if set: vector v's output starts at element vec_off_out[v]
   uint64_t scale_w;         // multiplier applied to every output when kScale
   uint64_t store_limit;     // only outputs with index < store_limit are stored
   uint32_t logN, logS, logC, lo_bits, root_shift;
@@ -265,8 +267,8 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     const uint64_t gt = blockIdx.x;  // grid = nvec << logTiles tiles
     const uint64_t vec = gt >> logTiles;
     const uint32_t tile = (uint32_t)(gt & ((1ull << logTiles) - 1));
-    const T *__restrict__ src = (const T *)a.src + vec * a.vec_stride_in;
-    T *__restrict__ dst = (T *)a.dst + vec * a.vec_stride_out;
+    const T *__restrict__ src = (const T *)a.src + (a.vec_off_in ? a.vec_off_in[vec] : vec * a.vec_stride_in);
+    T *__restrict__ dst = (T *)a.dst + (a.vec_off_out ? a.vec_off_out[vec] : vec * a.vec_stride_out);
     if constexpr (EPI == kEpiPrefix) {  // tiles whose p has no k' = p + P k < out_len
       if (((tile << logC) >> logS) >= (uint32_t)a.store_limit) return;
     }

@@ -55,6 +55,9 @@ struct XformOpts {
   const void *post_lo = nullptr;  // output-position multiplier tables (last pass), or null
   const void *post_hi = nullptr;
   uint32_t *nonzero_count = nullptr;  // counts nonzero outputs at index >= store_limit
+  const uint64_t *off_in = nullptr;   // device table: vector v's input at element off_in[v] (not in_stride)
+  const uint64_t *off_out = nullptr;  // device table: vector v's output at element off_out[v] (not out_stride)
+  void *scratch[2] = {nullptr, nullptr};  // caller-owned intermediates (batch x N elements each), else the context's
 };
 
 // Pinned staging ring of the host-buffer entry points (hostio.hip): kSlots

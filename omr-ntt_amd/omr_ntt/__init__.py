@@ -111,6 +111,8 @@ def lib():
             "omr_batch_execute": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_execute_graph": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_execute_plain": (i32, [_VP, _VP, _VP, _VP]),
+            "omr_batch_execute_grouped": (i32, [_VP, _VP, _VP, _VP]),
+            "omr_batch_execute_grouped_graph": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_set_max_log_radix": (i32, [i32]),
             "omr_num_schedule_classes": (i32, []),
             "omr_schedule_class": (i32, [i32, u64, u64, i32, Pi, Pi, Pi, Pi, ctypes.POINTER(u32)]),
@@ -401,7 +403,8 @@ class BatchPlan:
 
     def execute(self, x, out, mode="graph"):
         fn = {"graph": lib().omr_batch_execute_graph, "persistent": lib().omr_batch_execute,
-              "plain": lib().omr_batch_execute_plain}[mode]
+              "plain": lib().omr_batch_execute_plain, "grouped": lib().omr_batch_execute_grouped,
+              "grouped_graph": lib().omr_batch_execute_grouped_graph}[mode]
         _check(fn(self._p, _dptr(x), _dptr(out), _stream_ptr()), f"omr_batch_execute({mode})")
         return out
 

@@ -518,7 +518,7 @@ def test_batched_uneven_sizes_all_modes(fname):
     info = plan.info()
     assert info["total_elems"] == sum(1 << it[2] for it in items)
     outs = {}
-    for mode in ("persistent", "graph", "plain"):
+    for mode in ("persistent", "graph", "plain", "grouped", "grouped_graph"):
         out = torch.zeros_like(t_in)
         plan.execute(t_in, out, mode)
         outs[mode] = E.from_device(out, F)
@@ -529,12 +529,13 @@ def test_batched_uneven_sizes_all_modes(fname):
         for mode, got in outs.items():
             assert np.array_equal(got[oo:oo + n], exp), (mode, ln, inv)
     # graph replays with eager work in between stay bit-exact
-    for rep in range(3):
-        out = torch.zeros_like(t_in)
-        plan.execute(t_in, out, "graph")
-        torch.cuda.synchronize()
-        _ = (t_in.float().sum() if rep == 1 else None)
-        assert np.array_equal(E.from_device(out, F), outs["plain"]), rep
+    for mode in ("graph", "grouped_graph"):
+        for rep in range(3):
+            out = torch.zeros_like(t_in)
+            plan.execute(t_in, out, mode)
+            torch.cuda.synchronize()
+            _ = (t_in.float().sum() if rep == 1 else None)
+            assert np.array_equal(E.from_device(out, F), outs["plain"]), (mode, rep)
     plan.close()
 
 
@@ -571,13 +572,15 @@ def test_batched_in_place():
     x = inputs(F, total, 13)
     t = E.to_device(x, F)
     plan = E.BatchPlan(F, items)
-    plan.execute(t, t, "persistent")
-    got = E.from_device(t, F)
-    for (io, oo, ln, inv) in items:
-        n = 1 << ln
-        seg = x[io:io + n]
-        exp = O.intt(F.id, seg) if inv else O.ntt(F.id, seg)
-        assert np.array_equal(got[oo:oo + n], exp), (ln, inv)
+    for mode in ("persistent", "grouped"):
+        t = E.to_device(x, F)
+        plan.execute(t, t, mode)
+        got = E.from_device(t, F)
+        for (io, oo, ln, inv) in items:
+            n = 1 << ln
+            seg = x[io:io + n]
+            exp = O.intt(F.id, seg) if inv else O.ntt(F.id, seg)
+            assert np.array_equal(got[oo:oo + n], exp), (mode, ln, inv)
     plan.close()
 
 
@@ -601,6 +604,12 @@ def test_batched_config4_full_size():
     plan.execute(x, y_graph, "graph")
     plan.execute(x, y_plain, "plain")
     assert torch.equal(y_graph, y_plain)
+    y_grouped = torch.empty_like(x)
+    for _ in range(2):  # the capturing call, then a replay
+        y_grouped.zero_()
+        plan.execute(x, y_grouped, "grouped_graph")
+        assert torch.equal(y_grouped, y_plain)
+    del y_grouped
     inv_items = [(o, o, ln, 1 - d) for (o, _, ln, d) in items]
     plan_inv = E.BatchPlan(F, inv_items)
     back = torch.empty_like(x)
@@ -767,7 +776,7 @@ def test_random_structured_against_oracle():
         x = inputs(F, total, 7300 + case)
         t_in = E.to_device(x, F)
         plan = E.BatchPlan(F, items)
-        for mode in ("persistent", "graph", "plain"):
+        for mode in ("persistent", "graph", "plain", "grouped", "grouped_graph"):
             out = torch.zeros_like(t_in)
             plan.execute(t_in, out, mode)
             got = E.from_device(out, F)
