@@ -199,8 +199,10 @@ omr_status omr_batch_execute_graph(omr_batch_plan *plan, const void *d_in, void 
 omr_status omr_batch_execute_plain(omr_batch_plan *plan, const void *d_in, void *d_out,
                                    void *stream);
 /* Grouped form: the vectors of each (size, direction) as one batched
- * transform of the pass engine under that size's tuned schedule; the first
- * pass reads and the last pass writes through per-vector offset tables.
+ * transform of the pass engine; the first pass reads and the last pass
+ * writes through per-vector offset tables.  A group of 32 MB or more runs its
+ * size's default (coalescing-first) schedule, a smaller one the tuned table's
+ * (whose entries are single-vector timings).
  * The _graph variant captures the launches for a (d_in, d_out) pair and
  * replays them; its capturing call runs them eagerly as well. */
 omr_status omr_batch_execute_grouped(omr_batch_plan *plan, const void *d_in, void *d_out, void *stream);
