@@ -51,3 +51,20 @@ def test_schedule_classes_are_valid_decompositions():
 def test_features():
     x = E.features(E.Goldilocks, 1 << 24, 1 << 8, 16)
     assert np.allclose(x, [2 / 3, 24 / 32, 8 / 32, 4 / 16])
+
+
+def test_multipass_prefix_class_rule():
+    """Class 34: the decomposition with n1 = 2^min(ceil(log out_len) + 2,
+    log N - 6), fused over several passes; valid only when that n1 is at
+    least out_len and above 2^10.  The default class follows the auto rule."""
+    G = E.Goldilocks
+    n = 1 << 22
+    assert E.schedule_class(G, n, 1 << 12, 34) == {"method": 2, "log_n1": 14}
+    assert E.schedule_class(G, n, 1 << 16, 34) == {"method": 2, "log_n1": 16}
+    assert E.schedule_class(G, n, 1 << 9, 34) == {"method": 2, "log_n1": 11}
+    assert E.schedule_class(G, n, 1 << 17, 34) is None  # n1 would fall below out_len
+    assert E.schedule_class(G, 1 << 16, 1 << 10, 34) is None  # n1 = 2^10: the one-pass classes
+    assert E.default_class(G, n, 1 << 12) == 34
+    assert E.default_class(G, n, 1 << 18) == 35
+    assert E.default_class(G, n, 1 << 8) == 32  # one fused pass, n1 = 2^8
+    assert E.default_class(G, n, 4) == 30
