@@ -274,6 +274,13 @@ omr_status omr_tune_classes(omr_ctx *ctx, int field, uint64_t n, uint64_t out_le
                             float *times_us);
 /* Installs class cls as the tuned choice for (field, n, out_len). */
 omr_status omr_tune_set_class(int field, uint64_t n, uint64_t out_len, int cls);
+/* Installs class cls for `batch` vectors of the shape.  Full transforms of
+ * two or more vectors totalling 32 MB or more have a slot of their own in the
+ * table: the single-vector timings favour small cache-resident passes,
+ * which lose up to 40 % on such batches.  Calls of that size read only that
+ * slot and take the default schedule when it is empty.  Other shapes behave
+ * as with omr_tune_set_class.  omr_autotune_shape installs through this. */
+omr_status omr_tune_set_class_batch(int field, uint64_t n, uint64_t out_len, uint64_t batch, int cls);
 omr_status omr_tune_clear(void);
 /* Text table of the tuned choices (one shape per line). */
 omr_status omr_tune_save(const char *path);

@@ -675,7 +675,8 @@ omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void
   Schedule s;
   omr_status st = variant_schedule(f, variant, logN, s);
   if (st != OMR_OK) return st;
-  if (variant == OMR_ARK_RADIX2) tuned_schedule(f, logN, s);
+  if (variant == OMR_ARK_RADIX2 && !tuned_schedule_for(f, logN, batch, s) && batched_slot(f, logN, batch))
+    default_schedule(f, logN, s);
   XformOpts o;
   return run_transform(ctx, f, d_in, d_out, logN, omega, s, batch, N, N, o, stream);
 }
@@ -731,7 +732,7 @@ extern "C" omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in,
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
   const uint32_t logN = ilog2(n);
   Schedule s;
-  if (!tuned_schedule(field, logN, s)) default_schedule(field, logN, s);
+  if (!tuned_schedule_for(field, logN, batch, s)) default_schedule(field, logN, s);
   XformOpts o;
   o.scale = true;
   o.scale_c = finv_host(field, n % kP[field]);

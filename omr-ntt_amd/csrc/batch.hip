@@ -136,13 +136,12 @@ extern "C" omr_status omr_batch_plan_destroy(omr_batch_plan *p) try {
   return OMR_E_DEVICE;
 }
 
-// Schedule of a group: the tuned table holds single-vector timings, which
-// favour small cache-resident passes; a group of 32 MB or more streams from
-// HBM and takes the default (coalescing-first) schedule instead (config 4:
-// 10.5 ms with it, 13.5 ms with the single-vector table).
+// Schedule of a group: a group of 32 MB or more takes the tuned table's
+// batched slot or, if that is empty, the default (coalescing-first)
+// schedule, never the single-vector entry, whose small cache-resident passes
+// lose on such batches (config 4: 10.5 ms against 13.5 ms).
 static void group_schedule(int f, uint32_t log_n, uint32_t count, Schedule &s) {
-  const uint64_t bytes = ((uint64_t)count << log_n) * field_bytes(f);
-  if (bytes >= (32ull << 20) || !tuned_schedule(f, log_n, s)) default_schedule(f, log_n, s);
+  if (!tuned_schedule_for(f, log_n, count, s)) default_schedule(f, log_n, s);
 }
 
 extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_batch_item *items,

@@ -112,6 +112,10 @@ void default_schedule(int f, uint32_t logN, Schedule &s);
 omr_status variant_schedule(int f, int variant, uint32_t logN, Schedule &s);
 // Autotuned schedule for (field, logN) if the tuning table has one.
 bool tuned_schedule(int f, uint32_t logN, Schedule &s);
+// The tuned schedule for `batch` vectors: batches of 32 MB or more use a slot
+// of their own (false when it is empty: take the default schedule).
+bool tuned_schedule_for(int f, uint32_t logN, uint64_t batch, Schedule &s);
+bool batched_slot(int f, uint32_t logN, uint64_t batch);
 void set_tuned_schedule(int f, uint32_t logN, const Schedule &s);
 // Autotuned partial-output method for (field, logN, log out_len), if any.
 bool tuned_partial(int f, uint32_t logN, uint32_t logOut, int &method, uint32_t &log_n1);

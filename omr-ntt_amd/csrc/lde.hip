@@ -68,7 +68,7 @@ extern "C" omr_status omr_coset_lde_device(omr_ctx *ctx, int field, const void *
   if ((st = get_tables(ctx, field, logM, w, &tw)) != OMR_OK) return st;
   if ((st = get_tables(ctx, field, logM, shift, &sh)) != OMR_OK) return st;
   Schedule s;
-  if (!tuned_schedule(field, logM, s)) default_schedule(field, logM, s);
+  if (!tuned_schedule_for(field, logM, ncols, s)) default_schedule(field, logM, s);
   XformOpts o;
   o.zero_tail = true;
   o.log_valid = ilog2v(n);
@@ -106,7 +106,7 @@ extern "C" omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const voi
   if ((st = get_tables(ctx, field, logM, sinv, &sh)) != OMR_OK) return st;
   if (d_nonzero && (st = launch_fill_u32(d_nonzero, 1, 0, hs)) != OMR_OK) return st;
   Schedule s;
-  if (!tuned_schedule(field, logM, s)) default_schedule(field, logM, s);
+  if (!tuned_schedule_for(field, logM, ncols, s)) default_schedule(field, logM, s);
   XformOpts o;
   o.scale = true;
   o.scale_c = finv_host(field, M % field_modulus(field));

@@ -129,6 +129,7 @@ using namespace omr;
 extern "C" omr_status omr_tune_classes(omr_ctx *ctx, int field, uint64_t n, uint64_t out_len,
                                        uint64_t batch, int reps, float *times_us);
 extern "C" omr_status omr_tune_set_class(int field, uint64_t n, uint64_t out_len, int cls);
+extern "C" omr_status omr_tune_set_class_batch(int field, uint64_t n, uint64_t out_len, uint64_t batch, int cls);
 
 struct omr_cost_model {
   omr_ctx *ctx = nullptr;
@@ -434,7 +435,7 @@ extern "C" omr_status omr_autotune_shape(omr_ctx *ctx, omr_cost_model *cm, int u
       }
     if (chosen < 0) return OMR_E_SCHEDULE;
   }
-  if ((st = omr_tune_set_class(field, n, out_len, chosen)) != OMR_OK) return st;
+  if ((st = omr_tune_set_class_batch(field, n, out_len, batch, chosen)) != OMR_OK) return st;
   *cls = chosen;
   if (timed) *timed = ntimed;
   return OMR_OK;

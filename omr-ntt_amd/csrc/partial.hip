@@ -330,7 +330,7 @@ omr_status partial_batch_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out
     return prefix_fused(ctx, f, d_in, N, d_out, out_len, logN, omega, out_len, l, (uint32_t)batch, stream);
   // long prefixes: the full transform of every vector, its last pass storing k < out_len
   Schedule sch;
-  if (!tuned_schedule(f, logN, sch)) default_schedule(f, logN, sch);
+  if (!tuned_schedule_for(f, logN, batch, sch)) default_schedule(f, logN, sch);
   XformOpts o;
   o.store_limit = out_len;
   return run_transform(ctx, f, d_in, d_out, logN, omega, sch, batch, N, out_len, o, stream);

@@ -34,8 +34,13 @@
 namespace omr {
 
 static std::mutex g_tune_mu;
-static std::map<std::pair<int, uint32_t>, Schedule> g_tuned;
+static std::map<std::pair<int, uint32_t>, Schedule> g_tuned;        // single vectors
+static std::map<std::pair<int, uint32_t>, Schedule> g_tuned_batch;  // batches of >= 32 MB
 static std::map<std::tuple<int, uint32_t, uint32_t>, std::pair<int, uint32_t>> g_tuned_partial;
+
+bool batched_slot(int f, uint32_t logN, uint64_t batch) {
+  return batch >= 2 && ((batch << logN) * (uint64_t)field_bytes(f)) >= (32ull << 20);
+}
 
 bool tuned_schedule(int f, uint32_t logN, Schedule &s) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -45,9 +50,27 @@ bool tuned_schedule(int f, uint32_t logN, Schedule &s) {
   return true;
 }
 
+// Batches of 32 MB or more have a slot of their own: the single-vector
+// timings favour small cache-resident passes, which lose up to 40 % on such
+// batches.  An empty batched slot is not filled from the single one: the
+// caller's default (coalescing-first) schedule is the better guess there.
+bool tuned_schedule_for(int f, uint32_t logN, uint64_t batch, Schedule &s) {
+  if (!batched_slot(f, logN, batch)) return tuned_schedule(f, logN, s);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tuned_batch.find({f, logN});
+  if (it == g_tuned_batch.end()) return false;
+  s = it->second;
+  return true;
+}
+
 void set_tuned_schedule(int f, uint32_t logN, const Schedule &s) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[{f, logN}] = s;
+}
+
+static void set_tuned_schedule_batch(int f, uint32_t logN, const Schedule &s) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned_batch[{f, logN}] = s;
 }
 
 bool tuned_partial(int f, uint32_t logN, uint32_t logOut, int &method, uint32_t &log_n1) {
@@ -325,9 +348,32 @@ extern "C" omr_status omr_tune_set_class(int field, uint64_t n, uint64_t out_len
   return OMR_E_DEVICE;
 }
 
+// Installs class `cls` for `batch` vectors of (field, n, out_len): the
+// batched slot for full transforms of 32 MB or more, else as omr_tune_set_class.
+extern "C" omr_status omr_tune_set_class_batch(int field, uint64_t n, uint64_t out_len, uint64_t batch,
+                                               int cls) try {
+  if (!field_ok(field)) return OMR_E_FIELD;
+  if (n == 0 || (n & (n - 1)) || out_len == 0 || out_len > n || (out_len & (out_len - 1)))
+    return OMR_E_SIZE;
+  if (batch == 0) return OMR_E_ARG;
+  const uint32_t logN = 63 - __builtin_clzll(n);
+  if (out_len == n && batched_slot(field, logN, batch)) {
+    Schedule s;
+    if (!class_schedule(field, cls, logN, s)) return OMR_E_SCHEDULE;
+    set_tuned_schedule_batch(field, logN, s);
+    return OMR_OK;
+  }
+  return omr_tune_set_class(field, n, out_len, cls);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
+}
+
 extern "C" omr_status omr_tune_clear(void) try {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned.clear();
+  g_tuned_batch.clear();
   g_tuned_partial.clear();
   return OMR_OK;
 } catch (const std::bad_alloc &) {
@@ -337,16 +383,21 @@ extern "C" omr_status omr_tune_clear(void) try {
 }
 
 // Text checkpoint of the tuned tables: one line per entry,
-//   F <field> <logN> <logE> <npass> <logR...>   or   P <field> <logN> <logOut> <method> <log_n1>
+//   F <field> <logN> <logE> <npass> <logR...>   (single vectors),
+//   B <field> <logN> <logE> <npass> <logR...>   (batches of 32 MB or more), or
+//   P <field> <logN> <logOut> <method> <log_n1>
 extern "C" omr_status omr_tune_save(const char *path) try {
   if (!path) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   FILE *fp = fopen(path, "w");
   if (!fp) return OMR_E_ARG;
-  for (auto &kv : g_tuned) {
-    fprintf(fp, "F %d %u %d %zu", kv.first.first, kv.first.second, kv.second.logE, kv.second.logR.size());
-    for (int r : kv.second.logR) fprintf(fp, " %d", r);
-    fprintf(fp, "\n");
+  for (int slot = 0; slot < 2; slot++) {
+    for (auto &kv : slot ? g_tuned_batch : g_tuned) {
+      fprintf(fp, "%c %d %u %d %zu", slot ? 'B' : 'F', kv.first.first, kv.first.second, kv.second.logE,
+              kv.second.logR.size());
+      for (int r : kv.second.logR) fprintf(fp, " %d", r);
+      fprintf(fp, "\n");
+    }
   }
   for (auto &kv : g_tuned_partial)
     fprintf(fp, "P %d %u %u %d %u\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first),
@@ -367,7 +418,7 @@ extern "C" omr_status omr_tune_load(const char *path) try {
   omr_status st = OMR_OK;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   while (fscanf(fp, " %c", &kind) == 1) {
-    if (kind == 'F') {
+    if (kind == 'F' || kind == 'B') {
       int f, le;
       unsigned ln;
       size_t np;
@@ -391,7 +442,7 @@ extern "C" omr_status omr_tune_load(const char *path) try {
         st = OMR_E_ARG;
         break;
       }
-      g_tuned[{f, ln}] = s;
+      (kind == 'F' ? g_tuned : g_tuned_batch)[{f, ln}] = s;
     } else if (kind == 'P') {
       int f, m;
       unsigned ln, lo, l1;

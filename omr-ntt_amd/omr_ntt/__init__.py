@@ -119,6 +119,7 @@ def lib():
             "omr_default_class": (i32, [i32, u64, u64]),
             "omr_tune_classes": (i32, [_VP, i32, u64, u64, u64, i32, ctypes.POINTER(ctypes.c_float)]),
             "omr_tune_set_class": (i32, [i32, u64, u64, i32]),
+            "omr_tune_set_class_batch": (i32, [i32, u64, u64, u64, i32]),
             "omr_tune_clear": (i32, []),
             "omr_tune_save": (i32, [ctypes.c_char_p]),
             "omr_tune_load": (i32, [ctypes.c_char_p]),
@@ -460,8 +461,10 @@ def tune_classes(field, n, out_len=None, batch=1, reps=5):
     return np.array(t[:], dtype=np.float64)
 
 
-def tune_set_class(field, n, out_len, cls):
-    _check(lib().omr_tune_set_class(field.id, n, out_len, cls), "omr_tune_set_class")
+def tune_set_class(field, n, out_len, cls, batch=1):
+    """Installs class `cls` for `batch` vectors of the shape (batches of 32 MB
+    or more have a table slot of their own)."""
+    _check(lib().omr_tune_set_class_batch(field.id, n, out_len, batch, cls), "omr_tune_set_class_batch")
 
 
 def tune_clear():

@@ -4,6 +4,9 @@ autotuner (every schedule class of every shape timed on the GPU).
     python -m omr_ntt.tune_db --out omr-ntt_amd/tuning/mi355x.tune
 
 Full transforms: every field, log2 N = 4 .. min(26, two-adicity), batch 1.
+Batched full transforms (the table's slot for batches of 32 MB or more):
+log2 N = 10 .. min(26, two-adicity), as many vectors as make 64 MB (at least
+two, at most 65535).
 Partial prefixes: log2 N in {16, 18, 20, 22, 24}, every log2 out_len < log2 N.
 """
 import argparse
@@ -30,6 +33,13 @@ def build(out, max_log=26, reps=5, partial=True):
             report.append({"field": F.name, "log_n": ln, "out": ln, "class": c, "timed": timed,
                            "schedule": E.schedule_class(F, n, n, c)})
             print(f"{F.name} 2^{ln}: class {c} {report[-1]['schedule']}", flush=True)
+        for ln in range(10, min(max_log, F.TWO_ADICITY) + 1):
+            n = 1 << ln
+            batch = min(65535, max(2, (64 << 20) // (n * F.bytes)))
+            c, timed = E.autotune_shape(F, n, n, batch, use_model=False, reps=reps)
+            report.append({"field": F.name, "log_n": ln, "out": ln, "batch": batch, "class": c, "timed": timed,
+                           "schedule": E.schedule_class(F, n, n, c)})
+            print(f"{F.name} 2^{ln} x {batch}: class {c} {report[-1]['schedule']}", flush=True)
         if partial:
             for ln in (16, 18, 20, 22, 24):
                 if ln > F.TWO_ADICITY:

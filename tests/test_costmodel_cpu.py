@@ -68,3 +68,34 @@ def test_multipass_prefix_class_rule():
     assert E.default_class(G, n, 1 << 18) == 35
     assert E.default_class(G, n, 1 << 8) == 32  # one fused pass, n1 = 2^8
     assert E.default_class(G, n, 4) == 30
+
+
+def test_tuning_batched_slot_save_load(tmp_path):
+    """Batches of 32 MB or more have a table slot of their own: installing a
+    class for one leaves the single-vector entry alone, and both survive a
+    save / load round trip ('F' and 'B' lines)."""
+    import os
+    G = E.Goldilocks
+    n = 1 << 16
+    valid = [c for c in range(30) if E.schedule_class(G, n, n, c)]
+    c1, c2 = valid[0], valid[-1]
+    assert E.schedule_class(G, n, n, c1)["log_radix"] != E.schedule_class(G, n, n, c2)["log_radix"]
+    shipped = os.path.join(os.path.dirname(E.LIB_PATH), "tuning", "mi355x.tune")
+    try:
+        E.tune_clear()
+        E.tune_set_class(G, n, n, c1)
+        E.tune_set_class(G, n, n, c2, batch=64)  # 64 x 512 KiB = 32 MiB: the batched slot
+        assert E.default_schedule(G, n) == E.schedule_class(G, n, n, c1)["log_radix"]
+        p1, p2 = str(tmp_path / "a.tune"), str(tmp_path / "b.tune")
+        E.tune_save(p1)
+        lines = open(p1).read().split("\n")
+        assert any(l.startswith("F 2 16 ") for l in lines) and any(l.startswith("B 2 16 ") for l in lines)
+        E.tune_clear()
+        E.tune_load(p1)
+        E.tune_save(p2)
+        assert open(p1).read() == open(p2).read()
+        E.tune_set_class(G, n, n, c1, batch=8)  # 4 MiB: below the threshold, the single slot
+        assert E.default_schedule(G, n) == E.schedule_class(G, n, n, c1)["log_radix"]
+    finally:
+        E.tune_clear()
+        E.tune_load(shipped)
