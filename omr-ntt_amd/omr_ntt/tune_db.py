@@ -5,8 +5,9 @@ autotuner (every schedule class of every shape timed on the GPU).
 
 Full transforms: every field, log2 N = 4 .. min(26, two-adicity), batch 1.
 Batched full transforms (the table's slot for batches of 32 MB or more):
-log2 N = 10 .. min(26, two-adicity), as many vectors as make 64 MB (at least
-two, at most 65535).
+log2 N = 10 .. min(26, two-adicity), as many vectors as make 512 MB (at least
+two, at most 65535), so that the timed batch streams from HBM rather than the
+256 MB Infinity Cache.
 Partial prefixes: log2 N in {16, 18, 20, 22, 24}, every log2 out_len < log2 N.
 """
 import argparse
@@ -35,7 +36,7 @@ def build(out, max_log=26, reps=5, partial=True):
             print(f"{F.name} 2^{ln}: class {c} {report[-1]['schedule']}", flush=True)
         for ln in range(10, min(max_log, F.TWO_ADICITY) + 1):
             n = 1 << ln
-            batch = min(65535, max(2, (64 << 20) // (n * F.bytes)))
+            batch = min(65535, max(2, (512 << 20) // (n * F.bytes)))
             c, timed = E.autotune_shape(F, n, n, batch, use_model=False, reps=reps)
             report.append({"field": F.name, "log_n": ln, "out": ln, "batch": batch, "class": c, "timed": timed,
                            "schedule": E.schedule_class(F, n, n, c)})
