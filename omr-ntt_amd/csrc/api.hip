@@ -179,6 +179,11 @@ static const void *scaled_hi(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, 
   return t.hi_scaled;
 }
 
+static void free_tables(TwTables &t) {
+  for (void **p : {&t.lo, &t.hi, &t.root, &t.pyr, &t.mid, &t.hi_scaled})
+    if (*p) (void)hipFree(*p), *p = nullptr;
+}
+
 omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const TwTables **out) {
   auto key = std::make_tuple(f, logN, root);
   auto it = ctx->tables.find(key);
@@ -193,11 +198,7 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
         ++it2;
         continue;
       }
-      (void)hipFree(it2->second.lo);
-      (void)hipFree(it2->second.hi);
-      (void)hipFree(it2->second.root);
-      if (it2->second.mid) (void)hipFree(it2->second.mid);
-      if (it2->second.hi_scaled) (void)hipFree(it2->second.hi_scaled);
+      free_tables(it2->second);
       it2 = ctx->tables.erase(it2);
     }
   }
@@ -230,6 +231,10 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
     rt[i] = to_w_host(f, cur);
     cur = fmul_host(f, cur, step_rt);
   }
+  std::vector<uint64_t> pyr(2 * nroot);  // level L: w_(2^L)^m = rt[m 2^(log_rmax - L)]
+  pyr[0] = to_w_host(f, 1);
+  for (uint32_t L = 0; L <= t.log_rmax; L++)
+    for (uint64_t m = 0; m < (1ull << L); m++) pyr[(1ull << L) + m] = rt[m << (t.log_rmax - L)];
   auto upload = [&](std::vector<uint64_t> &v, void **dp) -> omr_status {
     std::vector<uint32_t> v32;
     const void *src = v.data();
@@ -258,7 +263,7 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
     // tables of a value that is not a root of unity (coset shifts) have none
     if (t.pow2_unit >= 192) t.pow2_unit = 0, t.log_m = 0;
   }
-  omr_status st;
+  omr_status st = OMR_OK;
   t.log_mid = std::min<uint32_t>(logN, kLogMid);
   if (t.log_mid > t.log_rmax) {  // mid table root^(j N/2^log_mid), j < 2^log_mid
     std::vector<uint64_t> mid(1ull << t.log_mid);
@@ -268,13 +273,18 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
       v = to_w_host(f, cur);
       cur = fmul_host(f, cur, step);
     }
-    if ((st = upload(mid, &t.mid)) != OMR_OK) return st;
+    st = upload(mid, &t.mid);
   } else {
     t.log_mid = t.log_rmax;
   }
-  if ((st = upload(lo, &t.lo)) != OMR_OK) return st;
-  if ((st = upload(hi, &t.hi)) != OMR_OK) return st;
-  if ((st = upload(rt, &t.root)) != OMR_OK) return st;
+  if (st == OMR_OK) st = upload(lo, &t.lo);
+  if (st == OMR_OK) st = upload(hi, &t.hi);
+  if (st == OMR_OK) st = upload(rt, &t.root);
+  if (st == OMR_OK) st = upload(pyr, &t.pyr);
+  if (st != OMR_OK) {  // nothing of a partial set is kept
+    free_tables(t);
+    return st;
+  }
   auto res = ctx->tables.emplace(key, t);
   *out = &res.first->second;
   return OMR_OK;
@@ -440,9 +450,8 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.dst = (void *)X[i + 1];
     a.tw_lo = tab->lo;
     a.tw_hi = tab->hi;
-    a.root = tab->root;
+    a.root_pyr = tab->pyr;
     a.lo_bits = tab->lo_bits;
-    a.root_shift = tab->log_rmax - logR;
     a.tw_mid = tab->mid;
     a.log_rmax = tab->log_rmax;
     a.log_mid = tab->log_mid;
@@ -541,13 +550,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) try {
   if (!ctx) return OMR_E_NULL;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
-  for (auto &kv : ctx->tables) {
-    (void)hipFree(kv.second.lo);
-    (void)hipFree(kv.second.hi);
-    (void)hipFree(kv.second.root);
-    if (kv.second.mid) (void)hipFree(kv.second.mid);
-    if (kv.second.hi_scaled) (void)hipFree(kv.second.hi_scaled);
-  }
+  for (auto &kv : ctx->tables) free_tables(kv.second);
   for (auto &b : ctx->scratch) free_buf(b);
   free_buf(ctx->io[0]);
   free_buf(ctx->io[1]);

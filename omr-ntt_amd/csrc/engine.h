@@ -24,7 +24,6 @@ struct PassArgs {
   void *dst;
   const void *tw_lo;   // w_N^e, e < 2^lo_bits                     (multiplier form)
   const void *tw_hi;   // w_N^(e 2^lo_bits), e < N / 2^lo_bits     (multiplier form)
-  const void *root;    // w_Rmax^m, m < Rmax                        (multiplier form)
   const void *pre_lo;  // optional input-position multiplier h^i:   h^(i mod 2^lo_bits)
   const void *pre_hi;  //                                            h^((i >> lo_bits) 2^lo_bits)
   const void *post_lo;  // optional output-position multiplier, same two-level layout
@@ -36,13 +35,14 @@ struct PassArgs {
   const uint64_t *vec_off_out;  // if set: vector v's output starts at element vec_off_out[v]
   uint64_t scale_w;         // multiplier applied to every output when kScale
   uint64_t store_limit;     // only outputs with index < store_limit are stored
-  uint32_t logN, logS, logC, lo_bits, root_shift;
+  uint32_t logN, logS, logC, lo_bits;
   uint32_t flags;     // kTwiddle | kScale | kZeroTail
   uint32_t logValid;  // with kZeroTail: input positions >= 2^logValid read as zero
   uint32_t nvec;              // vectors (consecutive, vec_stride_in/out apart)
   const void *tw_mid;         // w_N^(j N/2^log_mid), j < 2^log_mid (single-lookup twiddles)
-  uint32_t log_rmax;          // root table holds root_(2^log_rmax)^m
+  uint32_t log_rmax;          // largest level of the root pyramid
   uint32_t log_mid;
+  const void *root_pyr;  // root pyramid: w_(2^L)^m at [2^L + m], m < 2^L, L <= log_rmax (multiplier form)
 };
 enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u };
 
@@ -265,12 +265,14 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   const uint32_t logS = a.logS;
   const uint32_t S = 1u << logS;  // s <= N/R <= 2^31
   const bool staged = S < C;
-  const T *__restrict__ root = (const T *)a.root;
+  // the pass's roots w_R^m, m < R: one contiguous block of the root pyramid
+  const T *__restrict__ pyr = (const T *)a.root_pyr;
+  const T *__restrict__ rootR = pyr + R;
   const uint32_t logTiles = a.logN - LOGR - logC;
   const uint32_t nthr = C * RE;
 
   T wE[E / 2 + 1];
-  load_consts<F, LOGE, U>(wE, root, RE, a.root_shift);
+  load_consts<F, LOGE, U>(wE, rootR, RE, 0);
 
   {
     const uint64_t gt = blockIdx.x;  // grid = nvec << logTiles tiles
@@ -320,10 +322,10 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
 
     if (a.flags & kTwiddle) {  // w_N^(k p s) = w_(N/s)^(k p) on input row k
       const uint32_t logNs = a.logN - logS;
-      if (logNs <= a.log_mid) {  // one lookup: w_(N/s)^e from the root or the mid table
+      if (logNs <= a.log_mid) {  // one lookup: w_(N/s)^e from the root pyramid or the mid table
         const bool small = logNs <= a.log_rmax;
-        const T *tab = small ? root : (const T *)a.tw_mid;
-        const uint32_t sh = (small ? a.log_rmax : a.log_mid) - logNs;
+        const T *tab = small ? pyr + (1u << logNs) : (const T *)a.tw_mid;
+        const uint32_t sh = small ? 0u : a.log_mid - logNs;
 #pragma unroll
         for (int m = 0; m < E; m++) {
           const uint32_t e = (t + m * RE) * p;
@@ -343,7 +345,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
 
     // in-tile stage 0: stride R/E, one group per thread, no in-tile twiddle
     dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0, U>(x, wE);
-    tile_stage<F, LOGR, LOGE, 1, U>(x, wE, lds, root, t, c, logC, a.root_shift);
+    tile_stage<F, LOGR, LOGE, 1, U>(x, wE, lds, rootR, t, c, logC, 0);
 
     // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
     constexpr int EL = 1 << K::LOGLAST;
@@ -382,8 +384,9 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
       const uint64_t tps = (uint64_t)S >> logC;  // tiles per p
       T *__restrict__ part = (T *)a.dst + (gt >> logTiles) * tps * out_len + (j0 >> logC);
       const bool big = a.logN > a.log_mid;
-      const T *tab = a.logN <= a.log_rmax ? root : (const T *)a.tw_mid;
-      const uint32_t sh = (a.logN <= a.log_rmax ? a.log_rmax : a.log_mid) - a.logN;
+      const bool small = a.logN <= a.log_rmax;
+      const T *tab = small ? pyr + (1u << a.logN) : (const T *)a.tw_mid;
+      const uint32_t sh = small ? 0u : a.log_mid - a.logN;
       for (uint32_t r = tid; r < nrows; r += nthr) {
         T acc[4] = {0, 0, 0, 0};  // independent chains over the columns
         uint32_t cc = 0;

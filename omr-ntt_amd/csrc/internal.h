@@ -21,6 +21,10 @@ struct TwTables {
   void *lo = nullptr;    // root^e,               e < 2^lo_bits
   void *hi = nullptr;    // root^(e 2^lo_bits),   e < 2^(logN - lo_bits)
   void *root = nullptr;  // root^(m 2^(logN - log_rmax)), m < 2^log_rmax
+  // root pyramid: level L < log_rmax + 1 at [2^L, 2^(L+1)) holds w_(2^L)^m,
+  // m < 2^L (w_(2^L) = root^(2^(logN - L))), so the roots of every radix are
+  // one contiguous block
+  void *pyr = nullptr;
   uint32_t lo_bits = 0;
   uint32_t log_rmax = 0;
   void *mid = nullptr;     // root^(j 2^(logN - log_mid)), j < 2^log_mid: single-lookup twiddles

@@ -140,6 +140,8 @@ omr_status launch_field_op(int f, int op, const void *a, const void *b, void *ou
 
 hipError_t launch_pass(int field, int logR, int logE, int unit, const PassArgs &a, uint64_t total_tiles,
                        dim3 block, hipStream_t stream) {
+  // the pass reads its roots from the pyramid's level logR
+  if (!a.root_pyr || logR < 0 || (uint32_t)logR > a.log_rmax) return hipErrorInvalidValue;
   const int epi = pass_epilogue(a);
   switch (field) {
     case kBabyBear: return launch_pass_babybear(logR, logE, epi, a, total_tiles, block, stream);

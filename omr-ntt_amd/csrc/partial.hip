@@ -37,9 +37,8 @@ static omr_status first_pass(omr_ctx *ctx, int f, const void *in, void *out, uin
   a.dst = out;
   a.tw_lo = tab->lo;
   a.tw_hi = tab->hi;
-  a.root = tab->root;
+  a.root_pyr = tab->pyr;
   a.lo_bits = tab->lo_bits;
-  a.root_shift = tab->log_rmax - logR;
   a.tw_mid = tab->mid;
   a.log_rmax = tab->log_rmax;
   a.log_mid = tab->log_mid;
@@ -168,9 +167,8 @@ static omr_status prefix_fused(omr_ctx *ctx, int f, const void *d_in, uint64_t i
     a.dst = last ? ctx->scratch[2].p : ctx->scratch[i % 2].p;
     a.tw_lo = tab->lo;
     a.tw_hi = tab->hi;
-    a.root = tab->root;
+    a.root_pyr = tab->pyr;
     a.lo_bits = tab->lo_bits;
-    a.root_shift = tab->log_rmax - logR;
     a.tw_mid = tab->mid;
     a.log_rmax = tab->log_rmax;
     a.log_mid = tab->log_mid;
