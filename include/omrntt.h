@@ -108,7 +108,8 @@ omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n, 
 /* ---- device-resident entry points --------------------------------------- */
 /* Forward transform of `batch` vectors of n elements (consecutive in memory,
  * stride n) under `variant`.  d_in may equal d_out.  For partial variants
- * out_len = min(2s, n) elements are written per vector. */
+ * out_len = min(2s, n) elements are written per vector.  batch <= 65535
+ * here and in the other dense batched entry points (OMR_E_ARG beyond). */
 omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const void *d_in, void *d_out,
                           uint64_t n, uint64_t batch, uint64_t omega, uint64_t s, void *stream);
 

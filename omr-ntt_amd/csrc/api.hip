@@ -386,7 +386,7 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
   const int es = field_bytes(f);
   const uint64_t N = 1ull << logN;
   if (batch == 0) return OMR_OK;
-  if (batch > 65535) return OMR_E_ARG;
+  if (batch > kMaxBatch) return OMR_E_ARG;
   if (logN == 0) {  // size-1 transform: identity (times the scale)
     const TwTables *tt;
     (void)tt;
@@ -694,6 +694,7 @@ extern "C" omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const
   if (st != OMR_OK) return st;
   const uint32_t logN = ilog2(n);
   if (!omr_variant_supports(variant, logN)) return OMR_E_RADIX;
+  if (batch > kMaxBatch) return OMR_E_ARG;
   if (!ctx || !d_in || !d_out) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
@@ -730,6 +731,7 @@ extern "C" omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in,
                                       uint64_t n, uint64_t batch, uint64_t omega, void *stream) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
+  if (batch > kMaxBatch) return OMR_E_ARG;
   if (!ctx || !d_in || !d_out) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
@@ -754,6 +756,7 @@ extern "C" omr_status omr_ntt_schedule_device(omr_ctx *ctx, int field, const voi
                                               int npass, int log_e, void *stream) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
+  if (batch > kMaxBatch) return OMR_E_ARG;
   if (!log_radix) return OMR_E_NULL;
   if (npass < 1 || npass > 8) return OMR_E_SCHEDULE;
   {
@@ -909,7 +912,7 @@ extern "C" omr_status omr_ntt_batched(omr_ctx *ctx, int field, void *buf, uint64
                                       uint64_t omega) try {
   omr_status st = validate_domain(field, n, omega);
   if (st != OMR_OK) return st;
-  if (batch == 0) return OMR_E_ARG;
+  if (batch == 0 || batch > kMaxBatch) return OMR_E_ARG;
   if (!ctx || !buf) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(ctx->host_mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;

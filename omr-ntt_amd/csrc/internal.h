@@ -11,6 +11,10 @@
 
 namespace omr {
 
+// Vectors per call of the dense transform entry points (one pass launch
+// covers them all; larger batches are split by the caller).
+constexpr uint64_t kMaxBatch = 65535;
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;

@@ -215,7 +215,7 @@ omr_status omr::tune_classes_subset(omr_ctx *ctx, int field, uint64_t n, uint64_
   omr_field_info(field, nullptr, nullptr, &two_adicity);
   const uint32_t logN = 63 - __builtin_clzll(n), logOut = 63 - __builtin_clzll(out_len);
   if ((int)logN > two_adicity) return OMR_E_TWO_ADICITY;
-  if (batch == 0 || batch > 65535 || (out_len < n && batch != 1)) return OMR_E_ARG;
+  if (batch == 0 || batch > kMaxBatch || (out_len < n && batch != 1)) return OMR_E_ARG;
   if (reps < 1) reps = 1;
   if (!ctx) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(ctx->mu);

@@ -92,6 +92,17 @@ def test_validation_before_device_work():
     assert L.omr_intt(None, 2, None, 1024, w) == 1
     assert L.omr_ntt_batched(None, 2, None, 1024, 0, w) == 12
     assert L.omr_ntt_batched(None, 2, None, 1024, 3, w) == 1
+    # dense batches: at most 65535 vectors per call, checked before pointers
+    # (n * batch * 8 bytes would overflow 64 bits for batch >= 2^51 at n = 1024)
+    big = 1 << 60
+    assert L.omr_ntt_batched(None, 2, None, 1024, 65536, w) == 12
+    assert L.omr_ntt_batched(None, 2, None, 1024, big, w) == 12
+    assert L.omr_ntt_device(None, 2, 1, None, None, 1024, big, w, 0, None) == 12   # Dft variant
+    assert L.omr_ntt_device(None, 2, 8, None, None, 1024, 65536, w, 8, None) == 12  # partial variant
+    assert L.omr_ntt_device(None, 2, 0, None, None, 1024, 65535, w, 0, None) == 1
+    assert L.omr_intt_device(None, 2, None, None, 1024, 65536, w, None) == 12
+    lr = (ctypes.c_int * 2)(5, 5)
+    assert L.omr_ntt_schedule_device(None, 2, None, None, 1024, 65536, w, 0, lr, 2, 0, None) == 12
     assert L.omr_coset_lde(None, 0, None, None, 1 << 26, 1, 4, 0) == 3   # 2^28 > BabyBear 2-adicity
     assert L.omr_coset_lde(None, 9, None, None, 1024, 1, 4, 0) == 4
     assert L.omr_coset_lde(None, 2, None, None, 1024, 1, 3, 0) == 2
