@@ -221,10 +221,10 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 template <class F, int LOGR, int LOGE, int EPI>
 __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const typename F::T *__restrict__ src,
                                           const PassArgs &a, uint32_t tile, uint32_t tid, uint32_t c,
-                                          uint32_t t, bool staged) {
+                                          uint32_t t, bool staged, uint32_t logC) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
-  const uint32_t logC = a.logC, logS = a.logS;
+  const uint32_t logS = a.logS;
   uint32_t pos0, step;
   if (!staged) {
     const uint32_t q = (tile << logC) + c;
@@ -251,14 +251,17 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
 // grid-stride loop over several tiles per workgroup made the compiler keep
 // per-tile invariants live across iterations: 2-3x the VGPRs and a quarter
 // of the occupancy, measured slower.)
-template <class F, int LOGR, int LOGE, int U, int EPI>
+// FIXC: the tile has the full K::CMAX columns (a.logC == log2 CMAX, the case
+// of every pass over a vector of N >= R CMAX), known at compile time, so the
+// LDS exchange addresses fold into instruction offsets.
+template <class F, int LOGR, int LOGE, int U, int EPI, bool FIXC>
 __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   constexpr int R = K::R, E = K::E, RE = K::RE;
   __shared__ T lds[lds_padded(K::LDS_N)];
 
-  const uint32_t logC = a.logC;
+  const uint32_t logC = FIXC ? (uint32_t)__builtin_ctz(K::CMAX) : a.logC;
   const uint32_t C = 1u << logC;
   const uint32_t tid = threadIdx.x;
   const uint32_t c = tid & (C - 1), t = tid >> logC;
@@ -287,7 +290,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     // raised issue priority while the tile's loads go out: a wave starting
     // a tile gets its memory requests in ahead of waves in their arithmetic
     __builtin_amdgcn_s_setprio(1);
-    load_tile<F, LOGR, LOGE, EPI>(x, src, a, tile, tid, c, t, staged);
+    load_tile<F, LOGR, LOGE, EPI>(x, src, a, tile, tid, c, t, staged, logC);
     __builtin_amdgcn_s_setprio(0);
 
     const uint32_t q = (tile << logC) + c;
@@ -428,10 +431,10 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   }
 }
 
-template <class F, int LOGR, int LOGE, int U, int EPI>
+template <class F, int LOGR, int LOGE, int U, int EPI, bool FIXC>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, PassKernel<F, LOGR, LOGE>::MAXT)))
 k_stockham_pass(PassArgs a) {
-  stockham_pass_body<F, LOGR, LOGE, U, EPI>(a);
+  stockham_pass_body<F, LOGR, LOGE, U, EPI, FIXC>(a);
 }
 
 }  // namespace omr
