@@ -203,7 +203,11 @@ omr_status omr_batch_execute_plain(omr_batch_plan *plan, const void *d_in, void 
  * transform of the pass engine; the first pass reads and the last pass
  * writes through per-vector offset tables.  A group of 32 MB or more runs its
  * size's default (coalescing-first) schedule, a smaller one the tuned table's
- * (whose entries are single-vector timings).
+ * (whose entries are single-vector timings); the schedules are fixed when the
+ * plan is made.  The groups run on 2 library streams at once (OMR_BATCH_LANES
+ * = 1..4 when the plan is made changes that), forked
+ * from `stream` and joined back to it (work queued on `stream` afterwards
+ * sees every output).
  * The _graph variant captures the launches for a (d_in, d_out) pair and
  * replays them; its capturing call runs them eagerly as well. */
 omr_status omr_batch_execute_grouped(omr_batch_plan *plan, const void *d_in, void *d_out, void *stream);

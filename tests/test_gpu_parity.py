@@ -539,6 +539,35 @@ def test_batched_uneven_sizes_all_modes(fname):
     plan.close()
 
 
+@pytest.mark.parametrize("lanes", ["1", "3", "4"])
+def test_grouped_lanes(lanes, monkeypatch):
+    """The grouped form runs its (size, direction) groups on 1..4 streams
+    (OMR_BATCH_LANES, read when the plan is made): every lane count gives
+    the oracle's outputs, eagerly and over graph replays, in and out of place."""
+    monkeypatch.setenv("OMR_BATCH_LANES", lanes)
+    F = field("Goldilocks")
+    rng = np.random.default_rng(int(lanes))
+    items, total = _batch_items(rng, 60, 4, 15, layout_gap=1)
+    x = inputs(F, total, 21)
+    t_in = E.to_device(x, F)
+    plan = E.BatchPlan(F, items)
+    exp = x.copy()
+    for (io, oo, ln, inv) in items:
+        seg = x[io:io + (1 << ln)]
+        exp[oo:oo + (1 << ln)] = O.intt(F.id, seg) if inv else O.ntt(F.id, seg)
+    for rep in range(3):
+        for mode in ("grouped", "grouped_graph"):
+            out = t_in.clone()  # the gaps between vectors keep the input
+            plan.execute(t_in, out, mode)
+            torch.cuda.synchronize()
+            assert np.array_equal(E.from_device(out, F), exp), (lanes, mode, rep)
+    buf = t_in.clone()
+    plan.execute(buf, buf, "grouped")
+    torch.cuda.synchronize()
+    assert np.array_equal(E.from_device(buf, F), exp), (lanes, "in place")
+    plan.close()
+
+
 def test_batch_plan_survives_twiddle_cache_eviction():
     """A plan bakes twiddle-table pointers into its device task tables; the
     context's table cache (bounded at 128 entries) must not free them while
