@@ -111,8 +111,8 @@ def other_sizes(E, F, torch, np):
         xs = [E.to_device(np.random.default_rng(ln + i).integers(0, F.p, size=n, dtype=np.uint64), F)
               for i in range(nb)]
         ys = [E.empty_device(n, F) for _ in range(nb)]
-        fwd = time_device(lambda i: E.ntt_device(xs[i % nb], ys[i % nb], F), 30, 5, torch)
-        inv = time_device(lambda i: E.intt_device(ys[i % nb], xs[i % nb], F), 30, 5, torch)
+        fwd = time_device(lambda i: E.ntt_device(xs[i % nb], ys[i % nb], F), 200, 200, torch)
+        inv = time_device(lambda i: E.intt_device(ys[i % nb], xs[i % nb], F), 200, 200, torch)
         out[f"2^{ln}"] = {"forward_us": fwd * 1e6, "inverse_us": inv * 1e6, "schedule": E.default_schedule(F, n),
                           "elems_per_s": n / fwd}
         log(f"bench: 2^{ln} forward {fwd * 1e6:.1f} us inverse {inv * 1e6:.1f} us")
@@ -156,8 +156,8 @@ def other_fields(E, torch, np):
         xs = [E.to_device(np.random.default_rng(24 + i).integers(0, F.p, size=n, dtype=np.uint64), F)
               for i in range(3)]
         ys = [E.empty_device(n, F) for _ in range(3)]
-        fwd = time_device(lambda i: E.ntt_device(xs[i % 3], ys[i % 3], F), 20, 5, torch)
-        inv = time_device(lambda i: E.intt_device(ys[i % 3], xs[i % 3], F), 20, 5, torch)
+        fwd = time_device(lambda i: E.ntt_device(xs[i % 3], ys[i % 3], F), 100, 200, torch)
+        inv = time_device(lambda i: E.intt_device(ys[i % 3], xs[i % 3], F), 100, 200, torch)
         out[F.name] = {"forward_us": fwd * 1e6, "inverse_us": inv * 1e6, "elems_per_s": n / fwd,
                        "schedule": E.default_schedule(F, n)}
         log(f"bench: {F.name} 2^24 forward {fwd * 1e6:.1f} us inverse {inv * 1e6:.1f} us")
@@ -172,8 +172,8 @@ def config1(E, torch, np):
     n = 1 << 16
     x = E.to_device(np.random.default_rng(16).integers(0, F.p, size=n, dtype=np.uint64), F)
     y = E.empty_device(n, F)
-    fwd = time_device(lambda i: E.ntt_device(x, y, F), 200, 20, torch)
-    inv = time_device(lambda i: E.intt_device(y, x, F), 200, 20, torch)
+    fwd = time_device(lambda i: E.ntt_device(x, y, F), 200, 200, torch)
+    inv = time_device(lambda i: E.intt_device(y, x, F), 200, 200, torch)
     log(f"bench: BabyBear 2^16 forward {fwd * 1e6:.1f} us inverse {inv * 1e6:.1f} us")
     return {"forward_us": fwd * 1e6, "inverse_us": inv * 1e6, "schedule": E.default_schedule(F, n),
             "elems_per_s": n / fwd}
@@ -186,13 +186,13 @@ def config3(E, torch, np):
     for F in (E.BabyBear, E.Goldilocks):
         x = E.to_device(np.random.default_rng(22).integers(0, F.p, size=n, dtype=np.uint64), F)
         y = E.empty_device(n, F)
-        full = time_device(lambda i: E.ntt_device(x, y, F), 20, 3, torch)
+        full = time_device(lambda i: E.ntt_device(x, y, F), 100, 100, torch)
         rows = {"full_us": full * 1e6}
         for lo in (8, 12, 16, 20):
             m = 1 << lo
             o = E.empty_device(m, F)
-            pre = time_device(lambda i: E.partial_device(x, o, F, m, mode=E.PREFIX), 20, 3, torch)
-            strd = time_device(lambda i: E.partial_device(x, o, F, m, mode=E.STRIDE), 20, 3, torch)
+            pre = time_device(lambda i: E.partial_device(x, o, F, m, mode=E.PREFIX), 100, 100, torch)
+            strd = time_device(lambda i: E.partial_device(x, o, F, m, mode=E.STRIDE), 100, 100, torch)
             rows[f"2^{lo}"] = {"prefix_us": pre * 1e6, "stride_us": strd * 1e6}
         out[F.name] = rows
         log(f"bench: partial {F.name} 2^22: " + ", ".join(
@@ -223,8 +223,8 @@ def config5(E, torch, np, with_model=True):
     evals = torch.empty(m * cols, dtype=torch.int32, device="cuda")
     back = torch.empty_like(coeffs)
     nz = torch.zeros(1, dtype=torch.int32, device="cuda")
-    enc = time_device(lambda i: E.coset_lde_device(coeffs, evals, F, n, cols, blow), 5, 1, torch)
-    dec = time_device(lambda i: E.coset_decode_device(evals, back, F, n, cols, blow, nonzero=nz), 5, 1, torch)
+    enc = time_device(lambda i: E.coset_lde_device(coeffs, evals, F, n, cols, blow), 10, 5, torch)
+    dec = time_device(lambda i: E.coset_decode_device(evals, back, F, n, cols, blow, nonzero=nz), 10, 5, torch)
     ok = bool(torch.equal(back, coeffs)) and int(nz.item()) == 0
     if with_model:
         E.tune_clear()
@@ -257,10 +257,10 @@ def batched(E, F, torch, np):
     plan = E.BatchPlan(F, items)
     info = plan.info()
     y_grouped = torch.empty_like(x)
-    t_graph = time_device(lambda i: plan.execute(x, y_graph, "graph"), 5, 2, torch)
-    t_pers = time_device(lambda i: plan.execute(x, y_graph, "persistent"), 5, 1, torch)
-    t_grouped = time_device(lambda i: plan.execute(x, y_grouped, "grouped_graph"), 5, 2, torch)
-    t_plain = time_device(lambda i: plan.execute(x, y_plain, "plain"), 3, 1, torch)
+    t_graph = time_device(lambda i: plan.execute(x, y_graph, "graph"), 10, 3, torch)
+    t_pers = time_device(lambda i: plan.execute(x, y_graph, "persistent"), 10, 3, torch)
+    t_grouped = time_device(lambda i: plan.execute(x, y_grouped, "grouped_graph"), 10, 5, torch)
+    t_plain = time_device(lambda i: plan.execute(x, y_plain, "plain"), 3, 2, torch)
     same = bool(torch.equal(y_graph, y_plain)) and bool(torch.equal(y_grouped, y_plain))
     plan.close()
     best = min(t_graph, t_grouped)
@@ -430,8 +430,8 @@ def summarize_trace(trace_dir, logn=24):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--logn", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -511,7 +511,7 @@ def main():
         E.intt_device(ys[i % nbuf], xs[i % nbuf], F)
 
     # intt(ntt(x)) == x, so the inverse steps rewrite the inputs unchanged
-    sec_inv = time_device(inv, max(5, args.steps // 2), 3, torch)
+    sec_inv = time_device(inv, args.steps, args.warmup, torch)
     log(f"bench: inverse {sec_inv * 1e6:.1f} us/transform, round trip ok={round_trip}")
 
     alg_bytes = 16 * n
