@@ -184,7 +184,10 @@ typedef struct {
 typedef struct omr_batch_plan omr_batch_plan;
 
 /* Plans a batch: pass schedules per vector, tile task lists per pass,
- * twiddle tables and intermediate buffers. */
+ * twiddle tables and intermediate buffers.  Every execution of a plan uses
+ * the plan's intermediate buffers: executions queued on one stream run in
+ * order, but executions on different streams must not overlap in time (order
+ * them with an event), or they overwrite each other's intermediates. */
 omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_batch_item *items,
                                  uint32_t count, omr_batch_plan **out);
 omr_status omr_batch_plan_destroy(omr_batch_plan *plan);
