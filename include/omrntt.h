@@ -19,8 +19,10 @@
  *    leaves the library (host allocation failure: OMR_E_ALLOC).
  *  - `_device` entry points take device pointers and a hipStream_t (passed as
  *    void*, NULL = default stream) and are asynchronous on that stream.
- *    Calls sharing one omr_ctx share its scratch buffers: issue them on one
- *    stream (or use one context per stream).
+ *    Calls sharing one omr_ctx share its scratch buffers, so the context
+ *    orders them: a call on a stream other than the previous call's first
+ *    waits until the device is idle (calls on one stream are not slowed).
+ *    For concurrent streams use one context per stream.
  *  - The host-buffer entry point omr_ntt is synchronous and thread safe
  *    (it holds the context lock), matching `NttEncoder: Send + Sync`.
  */

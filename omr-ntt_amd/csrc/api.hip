@@ -699,6 +699,7 @@ extern "C" omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
   hipStream_t hs = (hipStream_t)stream;
+  StreamOrder ord(ctx, hs);
   if (!is_partial_variant(variant)) return forward_full(ctx, field, variant, d_in, d_out, logN, batch, omega, hs);
   const uint64_t out_len = std::min<uint64_t>(2 * s, n);
   if (out_len == 0) return OMR_OK;
@@ -735,6 +736,7 @@ extern "C" omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in,
   if (!ctx || !d_in || !d_out) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  StreamOrder ord(ctx, (hipStream_t)stream);
   const uint32_t logN = ilog2(n);
   Schedule s;
   if (!tuned_schedule_for(field, logN, batch, s)) default_schedule(field, logN, s);
@@ -774,6 +776,7 @@ extern "C" omr_status omr_ntt_schedule_device(omr_ctx *ctx, int field, const voi
   s.logE = log_e;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  StreamOrder ord(ctx, (hipStream_t)stream);
   const uint32_t logN = ilog2(n);
   XformOpts o;
   uint64_t root = omega;
@@ -805,6 +808,7 @@ extern "C" omr_status omr_partial_device(omr_ctx *ctx, int field, const void *d_
   if (!ctx || !d_in || !d_out) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  StreamOrder ord(ctx, (hipStream_t)stream);
   return partial_impl(ctx, field, d_in, d_out, ilog2(n), omega, out_len, mode, method, log_n1,
                       (hipStream_t)stream);
 } catch (const std::bad_alloc &) {
@@ -823,6 +827,7 @@ extern "C" omr_status omr_partial_batch_device(omr_ctx *ctx, int field, const vo
   if (!ctx || !d_in || !d_out) return OMR_E_NULL;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  StreamOrder ord(ctx, (hipStream_t)stream);
   return partial_batch_impl(ctx, field, d_in, d_out, ilog2(n), batch, omega, out_len, (hipStream_t)stream);
 } catch (const std::bad_alloc &) {
   return OMR_E_ALLOC;
@@ -854,6 +859,7 @@ extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, u
   const int es = field_bytes(field);
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  StreamOrder ord(ctx, nullptr);  // this form runs on the null stream
   if ((st = ensure_buf(ctx->io[0], (size_t)n * es)) != OMR_OK) return st;
   if ((st = host_to_device(ctx, ctx->io[0].p, buf, (size_t)n * es)) != OMR_OK) return st;
   uint64_t copy_back = n;

@@ -519,6 +519,7 @@ extern "C" omr_status omr_batch_execute_plain(omr_batch_plan *p, const void *d_i
   omr_ctx *ctx = p->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  StreamOrder ord(ctx, (hipStream_t)stream);  // the plain form runs through the context's scratch
   const int es = field_bytes(p->field);
   for (const omr_batch_item &it : p->items) {
     uint64_t w;

@@ -94,9 +94,38 @@ struct omr_ctx {
   std::mutex host_mu;
   omr::DevBuf hio[3];
   omr::HostStage stage;  // its own lock, taken last
+  // the stream of the last call that used the shared buffers above
+  // (omr::StreamOrder)
+  hipStream_t last_stream = nullptr;
+  bool has_last = false;
 };
 
 namespace omr {
+
+// The device entry points share the context's scratch and staging buffers,
+// and the context lock orders only their host side: two calls on different
+// streams could otherwise overlap on the GPU and overwrite each other's
+// intermediates.  So a call on a stream other than the previous call's first
+// waits until the device is idle (calls that stay on one stream cost a
+// pointer compare; a stream switch costs one device synchronisation).  A
+// stream under graph capture is left alone: a captured call is ordered by
+// whoever replays the graph.  Construct under ctx->mu.
+struct StreamOrder {
+  StreamOrder(omr_ctx *ctx, hipStream_t s) {
+    if (ctx->has_last && ctx->last_stream == s) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+      (void)hipGetLastError();
+      cs = hipStreamCaptureStatusNone;
+    }
+    if (cs != hipStreamCaptureStatusNone) return;
+    if (ctx->has_last) (void)hipDeviceSynchronize();
+    ctx->last_stream = s;
+    ctx->has_last = true;
+  }
+  StreamOrder(const StreamOrder &) = delete;
+  StreamOrder &operator=(const StreamOrder &) = delete;
+};
 
 int field_bytes(int f);
 bool field_ok(int f);

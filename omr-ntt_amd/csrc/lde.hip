@@ -62,6 +62,7 @@ extern "C" omr_status omr_coset_lde_device(omr_ctx *ctx, int field, const void *
   if (d_coeffs == d_evals) return OMR_E_ARG;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  StreamOrder ord(ctx, (hipStream_t)stream);
   uint64_t w;
   if ((st = omr_root_of_unity(field, 1ull << logM, &w)) != OMR_OK) return st;
   const TwTables *tw, *sh;
@@ -96,6 +97,7 @@ extern "C" omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const voi
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
   hipStream_t hs = (hipStream_t)stream;
+  StreamOrder ord(ctx, hs);
   uint64_t w;
   if ((st = omr_root_of_unity(field, 1ull << logM, &w)) != OMR_OK) return st;
   const uint64_t M = 1ull << logM;

@@ -252,6 +252,7 @@ omr_status omr::tune_classes_subset(omr_ctx *ctx, int field, uint64_t n, uint64_
   omr_root_of_unity(field, n, &w);
   hipStream_t s;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return OMR_E_DEVICE;
+  StreamOrder ord(ctx, s);
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);

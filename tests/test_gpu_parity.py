@@ -814,3 +814,54 @@ def test_random_structured_against_oracle():
                 want = O.intt(F.id, seg) if inv else O.ntt(F.id, seg)
                 assert np.array_equal(got[oo:oo + (1 << ln)], want), (case, mode, ln, inv)
         plan.close()
+
+
+def test_concurrent_host_threads_share_one_context():
+    """The reference trait is Send + Sync (src/encoder.rs:70): concurrent
+    calls on one engine must be safe.  Four host threads share the process's
+    context (ctypes drops the GIL during each call), each on its own stream,
+    mixing host-buffer calls (staging ring, table cache) with device calls
+    (forward, inverse, prefix) at different sizes; every result is checked
+    against the oracle."""
+    import threading
+    errors = []
+
+    def worker(k):
+        try:
+            rng = np.random.default_rng(100 + k)
+            stream = torch.cuda.Stream()
+            for it in range(6):
+                F = (E.Goldilocks, E.BabyBear)[(k + it) % 2]
+                ln = int(rng.integers(8, 19))
+                n = 1 << ln
+                x = inputs(F, n, 1000 * k + it)
+                if it % 3 == 0:  # host buffer: the trait path
+                    buf = x.astype(F.np_dtype)  # the field's word type
+                    E.ArkRadix2().ntt(buf, E.NttDomain(n, F))
+                    assert np.array_equal(buf.astype(np.uint64), O.ntt(F.id, x)), ("host", k, it, ln)
+                    continue
+                with torch.cuda.stream(stream):
+                    t = E.to_device(x, F)
+                    o = torch.empty_like(t)
+                    if it % 3 == 1:
+                        E.ntt_device(t, o, F)
+                        E.intt_device(o, t, F)
+                        stream.synchronize()
+                        assert np.array_equal(E.from_device(o, F), O.ntt(F.id, x)), ("fwd", k, it, ln)
+                        assert np.array_equal(E.from_device(t, F), x), ("inv", k, it, ln)
+                    else:
+                        m = 1 << int(rng.integers(2, ln))
+                        E.partial_device(t, o, F, m)
+                        stream.synchronize()
+                        assert np.array_equal(E.from_device(o, F)[:m], O.ntt(F.id, x)[:m]), ("prefix", k, it, ln, m)
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    E.context()
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    assert not any(th.is_alive() for th in threads), "a worker did not finish"
+    assert not errors, errors
