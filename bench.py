@@ -68,21 +68,22 @@ def cpu_baseline(x, n, budget_s=20.0):
 
 
 def cpu_extras(np, budget_s=4.0):
-    """The other configs' CPU baselines: the C restatements of the reference
+    """CPU baselines of the other workloads (BabyBear 2^16, Goldilocks 2^20 /
+    2^22 / 2^24, 2^22 prefixes): the C restatements of the reference
     encoders that run them, OpenMP over the host cores, median of up
     to 10 runs within a time budget per item (inputs built outside the
     timing, like criterion's iter_batched)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     items = [
-        ("config1_babybear_2p16_radix2_dit", O.BABYBEAR, 16, "ArkRadix2", None),
-        ("config2_goldilocks_2p20_winterfell", O.GOLDILOCKS, 20, "WinterfellFourStep", None),
-        ("config2_goldilocks_2p22_winterfell", O.GOLDILOCKS, 22, "WinterfellFourStep", None),
-        ("config2_goldilocks_2p24_stockham8", O.GOLDILOCKS, 24, "TfheStockhamRadix8", None),
-        ("config2_goldilocks_2p24_plonky3", O.GOLDILOCKS, 24, "Plonky3Radix2LayerSplit", None),
-        ("config3_babybear_2p22_td_out2p8", O.BABYBEAR, 22, "TransformDecomposition", 1 << 7),
-        ("config3_goldilocks_2p22_td_out2p8", O.GOLDILOCKS, 22, "TransformDecomposition", 1 << 7),
-        ("config3_babybear_2p22_td_out2p12", O.BABYBEAR, 22, "TransformDecomposition", 1 << 11),
+        ("babybear_2p16_radix2_dit", O.BABYBEAR, 16, "ArkRadix2", None),
+        ("goldilocks_2p20_winterfell", O.GOLDILOCKS, 20, "WinterfellFourStep", None),
+        ("goldilocks_2p22_winterfell", O.GOLDILOCKS, 22, "WinterfellFourStep", None),
+        ("goldilocks_2p24_stockham8", O.GOLDILOCKS, 24, "TfheStockhamRadix8", None),
+        ("goldilocks_2p24_plonky3", O.GOLDILOCKS, 24, "Plonky3Radix2LayerSplit", None),
+        ("babybear_2p22_td_out2p8", O.BABYBEAR, 22, "TransformDecomposition", 1 << 7),
+        ("goldilocks_2p22_td_out2p8", O.GOLDILOCKS, 22, "TransformDecomposition", 1 << 7),
+        ("babybear_2p22_td_out2p12", O.BABYBEAR, 22, "TransformDecomposition", 1 << 11),
     ]
     res = {}
     for key, f, logn, name, s in items:
@@ -166,7 +167,7 @@ def other_fields(E, torch, np):
     return out
 
 
-def config1(E, torch, np):
+def small_babybear(E, torch, np):
     """BabyBear 2^16 forward + inverse, latency per transform."""
     F = E.BabyBear
     n = 1 << 16
@@ -179,7 +180,7 @@ def config1(E, torch, np):
             "elems_per_s": n / fwd}
 
 
-def config3(E, torch, np):
+def partial_2p22(E, torch, np):
     """Partial outputs of N = 2^22: prefix (out_len) and stride subset."""
     out = {}
     n = 1 << 22
@@ -201,7 +202,7 @@ def config3(E, torch, np):
     return out
 
 
-def config5(E, torch, np, with_model=True):
+def encoder(E, torch, np, with_model=True):
     """256 BabyBear columns x 2^20 coefficients -> blowup 4 coset LDE
     (n = 2^22) -> decode, with the schedule chosen by the learned cost model
     (trained live on autotuner timings; held-out regret reported), or with
@@ -437,9 +438,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 128 MiB device copy (torch copy_) 10 times, to calibrate "
                          "rocprof FETCH_SIZE / WRITE_SIZE against a kernel of known traffic")
-    ap.add_argument("--extras", default="host,sizes,fields,batched,config1,config3,config5,costmodel",
+    ap.add_argument("--extras", default="host,sizes,fields,batched,small,partial,encoder,costmodel",
                     help="comma-separated legs after the headline (costmodel: train the cost model "
-                         "and pick config 5's schedule with it)")
+                         "and pick the 256 x 2^20 encoder's schedule with it)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 2^20/2^22 sizes and the batched uneven-size workload")
     ap.add_argument("--summarize-profiles", metavar="DIR",
@@ -569,18 +570,18 @@ def main():
     if "batched" in legs:
         result["batched"] = batched(E, F, torch, np)
         torch.cuda.empty_cache()
-    if "config1" in legs:
-        result["config1_babybear_2p16"] = config1(E, torch, np)
-    if "config3" in legs:
-        result["config3_partial_2p22"] = config3(E, torch, np)
+    if "small" in legs:
+        result["babybear_2p16"] = small_babybear(E, torch, np)
+    if "partial" in legs:
+        result["partial_2p22"] = partial_2p22(E, torch, np)
         torch.cuda.empty_cache()
-    if "config5" in legs:
-        result["config5_encoder"] = config5(E, torch, np, "costmodel" in legs)
+    if "encoder" in legs:
+        result["encoder_256x2p20"] = encoder(E, torch, np, "costmodel" in legs)
     if not args.no_cpu:
         cb, cpu_out = cpu_baseline(host_x, n)
         result["cpu_baseline"] = cb
         if not args.no_extras:
-            result["cpu_baselines_other_configs"] = cpu_extras(np)
+            result["cpu_baselines_other_workloads"] = cpu_extras(np)
         x0 = E.to_device(host_x, F)
         y0 = E.empty_device(n, F)
         E.ntt_device(x0, y0, F)

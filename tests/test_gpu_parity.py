@@ -614,7 +614,7 @@ def test_batched_in_place():
 
 
 @pytest.mark.slow
-def test_batched_config4_full_size():
+def test_batched_4096_uneven_full_size():
     """4096 Goldilocks vectors, log2 n ~ U{10..20} (seed 42), mixed directions:
     graph == plain bit for bit, forward-then-inverse round trip restores the
     input, and sampled vectors match the oracle."""
