@@ -156,7 +156,7 @@ extern "C" omr_status omr_batch_plan_destroy(omr_batch_plan *p) try {
 // Schedule of a group: a group of 32 MB or more takes the tuned table's
 // batched slot or, if that is empty, the default (coalescing-first)
 // schedule, never the single-vector entry, whose small cache-resident passes
-// lose on such batches (config 4: 10.5 ms against 13.5 ms).
+// lose on such batches (4096 Goldilocks vectors, log n ~ U{10..20}: 10.5 ms against 13.5 ms).
 static void group_schedule(int f, uint32_t log_n, uint32_t count, Schedule &s) {
   if (!tuned_schedule_for(f, log_n, count, s)) default_schedule(f, log_n, s);
 }
@@ -166,7 +166,7 @@ static void group_schedule(int f, uint32_t log_n, uint32_t count, Schedule &s) {
 static omr_status assign_lanes(omr_batch_plan *p, int es) {
   auto &gs = p->groups;
   if (gs.empty()) return OMR_OK;
-  // 2 lanes by default: config-4-like batches (log n ~ U{10..20}) gain 3 %
+  // 2 lanes by default: batches of 4096 vectors with log n ~ U{10..20} gain 3 %
   // over one lane, and 4 lanes no more; OMR_BATCH_LANES=1..4 overrides
   // (read per plan)
   uint32_t lanes = 2;

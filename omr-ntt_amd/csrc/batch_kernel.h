@@ -172,7 +172,7 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
 
 // 4 waves per SIMD: what the LDS tile allows (4 workgroups of 4 waves per
 // CU).  Goldilocks needs a few dwords of spill for it and still runs 7 %
-// faster than at 3 waves per SIMD without the cap (config-4 batch, measured).
+// faster than at 3 waves per SIMD without the cap (4096 Goldilocks vectors of log n ~ U{10..20}, measured).
 template <class F>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, kTileThreads), amdgpu_waves_per_eu(4)))
 k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchVec *__restrict__ vecs,

@@ -213,7 +213,7 @@ def test_custom_omega_is_honoured():
 
 
 @pytest.mark.parametrize("logn", [20, 22, 24])
-def test_goldilocks_large_forward_inverse(logn):  # config 2
+def test_goldilocks_large_forward_inverse(logn):  # Goldilocks 2^20 / 2^22 / 2^24
     F = E.Goldilocks
     n = 1 << logn
     x = inputs(F, n, 42)
@@ -234,7 +234,7 @@ def test_goldilocks_large_forward_inverse(logn):  # config 2
 
 
 @pytest.mark.parametrize("fname", ["BabyBear", "Goldilocks"])
-def test_partial_large_prefix_and_stride(fname):  # config 3
+def test_partial_large_prefix_and_stride(fname):  # N = 2^22, out_len 2^8 .. 2^20
     F = field(fname)
     n = 1 << 22
     x = inputs(F, n, 42)
@@ -425,7 +425,7 @@ def test_encoder_registry_names():
 
 
 @pytest.mark.parametrize("fname", FIELD_IDS)
-def test_coset_lde_golden_and_decode(fname):  # config 5 semantics at golden sizes
+def test_coset_lde_golden_and_decode(fname):  # the column encoder's semantics at golden sizes
     F = field(fname)
     for case in load("lde.json")[fname.lower()]:
         n, blow = case["n"], case["blowup"]
