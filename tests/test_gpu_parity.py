@@ -699,17 +699,17 @@ def test_maximum_sizes(fname):
     torch.cuda.empty_cache()
 
 
-def test_random_shapes_against_oracle():
+def _random_sweep(seed, cases, max_ln, max_batch):
     """Seeded sweep over shapes the fixed tests do not enumerate: every field,
-    sizes 2^1..2^14, every full encoder where it applies, batches, prefix and
-    stride-subset lengths (ragged prefixes included), explicit methods and
+    sizes 2^1..2^max_ln, every full encoder where it applies, batches, prefix
+    and stride-subset lengths (ragged prefixes included), explicit methods and
     sub-NTT sizes, explicit schedules and register radices, inverse."""
-    rng = np.random.default_rng(2024)
+    rng = np.random.default_rng(seed)
     full = [E.ARK_RADIX2, E.DFT, E.LAMBDA_BOWERS, E.LAMBDA_RADIX4, E.PLONKY3_LAYER_SPLIT,
             E.WINTERFELL_FOUR_STEP, E.TFHE_STOCKHAM_RADIX8]
-    for case in range(250):
+    for case in range(cases):
         F = E.FIELDS[int(rng.integers(0, 4))]
-        ln = int(rng.integers(1, 15))
+        ln = int(rng.integers(1, max_ln + 1))
         n = 1 << ln
         x = inputs(F, n, 5000 + case)
         X = O.ntt(F.id, x)
@@ -719,7 +719,7 @@ def test_random_shapes_against_oracle():
             out = E.empty_device(n, F)
             E.ntt_device(t, out, F, var)
             assert np.array_equal(E.from_device(out, F), X), (case, F.name, ln, var)
-        b = int(rng.integers(1, 5))
+        b = int(rng.integers(1, max_batch + 1))
         xb = inputs(F, n * b, 6000 + case)
         tb = E.to_device(xb, F)
         ob = E.empty_device(n * b, F)
@@ -729,6 +729,8 @@ def test_random_shapes_against_oracle():
             assert np.array_equal(gb[i * n:(i + 1) * n], O.ntt(F.id, xb[i * n:(i + 1) * n])), (case, i)
         out_len = int(rng.integers(1, n + 1))
         meth = int(rng.integers(0, 4))
+        if ln > 14 and meth in (1, 2):  # direct sums and tiny sub-NTTs cost out_len x N
+            meth = 0
         l1 = int(rng.integers(0, ln + 1)) if meth == 2 else 0
         po = E.empty_device(out_len, F)
         try:
@@ -754,6 +756,15 @@ def test_random_shapes_against_oracle():
             assert np.array_equal(E.from_device(so2, F), X), (case, F.name, parts, le)
             E.ntt_schedule_device(so2, so2, F, parts, log_e=le if le != 1 else 0, inverse=True)
             assert torch.equal(so2, t), (case, F.name, parts, le)
+
+
+def test_random_shapes_against_oracle():
+    _random_sweep(2024, 250, 14, 4)
+
+
+def test_random_large_shapes_against_oracle():
+    """Sizes up to 2^20, where every radix runs its full-column kernels."""
+    _random_sweep(7777, 80, 20, 2)
 
 
 def lib_supports(variant, log_n):
