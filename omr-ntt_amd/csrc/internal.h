@@ -4,6 +4,7 @@
 
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -97,6 +98,7 @@ struct omr_ctx {
   // the stream of the last call that used the shared buffers above
   // (omr::StreamOrder)
   hipStream_t last_stream = nullptr;
+  std::thread::id last_thread;  // for hipStreamPerThread, one stream per thread
   bool has_last = false;
 };
 
@@ -112,7 +114,8 @@ namespace omr {
 // whoever replays the graph.  Construct under ctx->mu.
 struct StreamOrder {
   StreamOrder(omr_ctx *ctx, hipStream_t s) {
-    if (ctx->has_last && ctx->last_stream == s) return;
+    const std::thread::id me = std::this_thread::get_id();
+    if (ctx->has_last && ctx->last_stream == s && (s != hipStreamPerThread || ctx->last_thread == me)) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
       (void)hipGetLastError();
@@ -121,6 +124,7 @@ struct StreamOrder {
     if (cs != hipStreamCaptureStatusNone) return;
     if (ctx->has_last) (void)hipDeviceSynchronize();
     ctx->last_stream = s;
+    ctx->last_thread = me;
     ctx->has_last = true;
   }
   StreamOrder(const StreamOrder &) = delete;
