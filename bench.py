@@ -452,6 +452,11 @@ def main():
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="no GPU work: print the kernel-duration table of a rocprofv3 --kernel-trace run under DIR")
     args = ap.parse_args()
+    unknown = set(args.extras.split(",")) - {"host", "sizes", "fields", "batched", "small", "partial",
+                                              "encoder", "costmodel"}
+    if unknown and not args.no_extras:
+        print(f"bench.py: unknown --extras legs {sorted(unknown)}", file=sys.stderr)
+        sys.exit(2)
     if args.summarize_profiles:
         summarize_profiles(args.summarize_profiles, args.profile_out, args.logn)
         return
