@@ -127,3 +127,23 @@ def test_default_schedules():
     assert E.default_schedule(E.BabyBear, 1 << 16) == [8, 8]
     assert sum(E.default_schedule(E.Goldilocks, 1 << 20)) == 20
     assert E.default_schedule(E.Goldilocks, 1 << 12) == [12]
+
+
+def test_missing_library_fails_loudly():
+    """No CPU fallback: without the HIP library every entry point raises."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import numpy as np, omr_ntt as E\n"
+        "E.LIB_PATH = '/nonexistent/libomrntt.so'\n"
+        "for f in (lambda: E.lib(), lambda: E.ArkRadix2().ntt(np.zeros(16, np.uint32), E.NttDomain(16, E.BabyBear))):\n"
+        "    try:\n"
+        "        f()\n"
+        "    except ImportError as e:\n"
+        "        assert 'missing' in str(e), e\n"
+        "    else:\n"
+        "        raise SystemExit('no error')\n"
+        "print('raised')\n" % os.path.join(ROOT, "omr-ntt_amd"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "raised" in r.stdout, r.stderr
