@@ -218,9 +218,14 @@ omr_status omr_batch_execute_plain(omr_batch_plan *plan, const void *d_in, void 
 omr_status omr_batch_execute_grouped(omr_batch_plan *plan, const void *d_in, void *d_out, void *stream);
 omr_status omr_batch_execute_grouped_graph(omr_batch_plan *plan, const void *d_in, void *d_out,
                                            void *stream);
-/* Largest pass radix (2^4 .. 2^12) the batch planner gives multi-pass vectors
- * of later plans (default 2^9). */
+/* Largest pass radix (2^5 .. 2^12; OMR_E_SCHEDULE otherwise) the batch
+ * planner gives multi-pass vectors of later plans (default 2^9). */
 omr_status omr_batch_set_max_log_radix(int log_radix);
+/* Pass radices the batch planner gives one vector of 2^log_n elements under
+ * a radix cap of 2^max_log_radix (0 = the current setting): one pass up to
+ * 2^12, else ceil(log_n / cap) balanced passes, each 2^4 .. 2^12.  Writes up
+ * to 9 log radices and the pass count.  Host only (no GPU work). */
+omr_status omr_batch_vector_schedule(uint32_t log_n, int max_log_radix, int *log_radix, int *npass);
 
 /* Elementwise device field arithmetic on canonical arrays (test hook):
  * op 0 add, 1 sub, 2 mul, 3 multiplier-form round trip of a, 4 a * b via

@@ -135,6 +135,24 @@ void free_buf(DevBuf &b) {
   b.bytes = 0;
 }
 
+// Goldilocks: the discrete log base 2 of the root of order min(N, 64) (the
+// register-DFT constants of the pass kernels are then compile-time shifts);
+// other fields, and values that are not roots of unity (coset shifts): none.
+void pow2_unit_of(int f, uint32_t logN, uint64_t root, TwTables &t) {
+  t.log_m = 0;
+  t.pow2_unit = 0;
+  if (f != kGoldilocks) return;
+  const uint32_t log_m = std::min<uint32_t>(logN, 6);
+  const uint64_t wm = fpow_host(f, root, 1ull << (logN - log_m));
+  uint64_t pw = 1;
+  for (uint32_t k = 0; k < 192; k++, pw = fmul_host(f, pw, 2))
+    if (pw == wm) {
+      t.pow2_unit = k;
+      t.log_m = log_m;
+      return;
+    }
+}
+
 // Two-level power table of `root` (order 2^logN) plus the radix root table.
 void pin_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, int delta) {
   auto it = ctx->tables.find(std::make_tuple(f, logN, root));
@@ -250,19 +268,7 @@ omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const T
     if (hipMemcpy(*dp, src, v.size() * es, hipMemcpyHostToDevice) != hipSuccess) return OMR_E_DEVICE;
     return OMR_OK;
   };
-  if (f == kGoldilocks) {  // discrete log base 2 of the root of order min(N, 64)
-    t.log_m = std::min<uint32_t>(logN, 6);
-    const uint64_t wm = fpow_host(f, root, 1ull << (logN - t.log_m));
-    uint64_t pw = 1;
-    t.pow2_unit = 1000;
-    for (uint32_t k = 0; k < 192; k++, pw = fmul_host(f, pw, 2))
-      if (pw == wm) {
-        t.pow2_unit = k;
-        break;
-      }
-    // tables of a value that is not a root of unity (coset shifts) have none
-    if (t.pow2_unit >= 192) t.pow2_unit = 0, t.log_m = 0;
-  }
+  pow2_unit_of(f, logN, root, t);
   omr_status st = OMR_OK;
   t.log_mid = std::min<uint32_t>(logN, kLogMid);
   if (t.log_mid > t.log_rmax) {  // mid table root^(j N/2^log_mid), j < 2^log_mid
@@ -648,6 +654,24 @@ static uint64_t next_pow2(uint64_t x) {
   return r;
 }
 
+// The partial encoders' output count min(2s, N) (transform_decomposition.rs:30,
+// dft.rs:43), without wrapping 2s for s >= 2^63.
+static uint64_t partial_out_len(uint64_t s, uint64_t n) { return s > n / 2 ? n : std::min<uint64_t>(2 * s, n); }
+
+// TransformDecomposition(V2)'s sub-NTT size, one rule for the host drop-in
+// (omr_ntt) and the device form (omr_ntt_device): the reference's
+// n1 = nextpow2(2 out_len) clamped to [2, N] (transform_decomposition.rs:38-39,
+// transform_decomposition_v2.rs:36); where the sub-NTTs cannot run as fused
+// passes, n1 is capped at the largest pass radix.  Any n1 gives the same
+// outputs (w_n1^(jk) = w_n1^(j (k mod n1))); the rule fixes the kernels run.
+static uint32_t td_log_n1(int es, uint32_t logN, uint64_t out_len) {
+  const uint64_t n = 1ull << logN;
+  const uint64_t n1 = std::min<uint64_t>(std::max<uint64_t>(next_pow2(2 * out_len), 2), n);  // out_len <= n <= 2^32
+  uint32_t log_n1 = ilog2(n1);
+  if (!decomposition_ok(es, logN, log_n1, out_len)) log_n1 = std::min<uint32_t>(log_n1, (uint32_t)max_log_radix(es));
+  return log_n1;
+}
+
 namespace omr {
 
 // Full forward transform of `batch` dense vectors under `variant`.
@@ -701,20 +725,11 @@ extern "C" omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const
   hipStream_t hs = (hipStream_t)stream;
   StreamOrder ord(ctx, hs);
   if (!is_partial_variant(variant)) return forward_full(ctx, field, variant, d_in, d_out, logN, batch, omega, hs);
-  const uint64_t out_len = std::min<uint64_t>(2 * s, n);
+  const uint64_t out_len = partial_out_len(s, n);
   if (out_len == 0) return OMR_OK;
   const int es = field_bytes(field);
-  int method = 2;
-  uint32_t log_n1 = 0;
-  if (variant == OMR_DFT_PARTIAL) method = 1;
-  else {
-    // reference rule: n1 = nextpow2(2 out_len) clamped to [2, N]
-    // (transform_decomposition.rs:38-39); where the sub-NTTs cannot run as
-    // fused passes (n1 > N/4), capped at the largest pass radix
-    uint64_t n1 = std::min<uint64_t>(std::max<uint64_t>(next_pow2(2 * out_len), 2), n);
-    log_n1 = ilog2(n1);
-    if (!decomposition_ok(es, logN, log_n1, out_len)) log_n1 = std::min<uint32_t>(log_n1, (uint32_t)max_log_radix(es));
-  }
+  const int method = variant == OMR_DFT_PARTIAL ? 1 : 2;
+  const uint32_t log_n1 = method == 2 ? td_log_n1(es, logN, out_len) : 0;
   for (uint64_t b = 0; b < batch; b++) {
     const char *in = (const char *)d_in + b * n * es;
     char *out = (char *)d_out + b * n * es;
@@ -867,15 +882,11 @@ extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, u
   if (!is_partial_variant(variant)) {
     st = forward_full(ctx, field, variant, ctx->io[0].p, ctx->io[0].p, logN, 1, omega, nullptr);
   } else {
-    copy_back = std::min<uint64_t>(2 * s, n);
+    copy_back = partial_out_len(s, n);
     if (copy_back == 0) return OMR_OK;
     if ((st = ensure_buf(ctx->io[1], (size_t)copy_back * es)) != OMR_OK) return st;
-    int method = variant == OMR_DFT_PARTIAL ? 1 : 2;
-    uint32_t log_n1 = 0;
-    if (method == 2) {
-      uint64_t n1 = std::min<uint64_t>(std::max<uint64_t>(next_pow2(2 * copy_back), 2), n);
-      log_n1 = std::min<uint32_t>(ilog2(n1), (uint32_t)max_log_radix(es));
-    }
+    const int method = variant == OMR_DFT_PARTIAL ? 1 : 2;
+    const uint32_t log_n1 = method == 2 ? td_log_n1(es, logN, copy_back) : 0;
     st = partial_impl(ctx, field, ctx->io[0].p, ctx->io[1].p, logN, omega, copy_back, 0, method,
                       log_n1, nullptr);
     result = ctx->io[1].p;

@@ -95,24 +95,50 @@ static hipError_t launch_batch(int f, dim3 grid, hipStream_t s, const BatchTask 
 // keeps >= 8 columns (64 B runs for Goldilocks) per 4096-element tile.
 static int g_batch_max_log_radix = 9;
 
-// Pass radices of one vector of size 2^logn inside a batch.
-static void batch_schedule(uint32_t logn, std::vector<int> &out) {
+// Pass radices of one vector of size 2^logn inside a batch, for a radix cap
+// of 2^cap: one pass up to 2^12, else ceil(logn / cap) balanced passes.
+// k_batch_tiles runs a pass of radix 2^4 .. 2^12 (register radix 16), and
+// with cap >= 5 every balanced part of a log n >= 13 is >= 4: with
+// parts = ceil(logn / 5) <= (logn + 4) / 5, logn / parts >= 5 logn / (logn + 4) >= 4
+// for logn >= 16, and logn = 13..15 split into three parts of 4 or 5.
+static bool batch_schedule(uint32_t logn, int cap, std::vector<int> &out) {
   out.clear();
   if (logn <= 12) {
     out.push_back((int)logn);
-    return;
+    return logn >= 4;
   }
-  const int cap = g_batch_max_log_radix;
+  if (cap < 5 || cap > 12) return false;
   const int parts = ((int)logn + cap - 1) / cap;
   const int base = (int)logn / parts, rem = (int)logn % parts;
   for (int i = 0; i < parts; i++) out.push_back(base + (i < rem ? 1 : 0));
+  for (int r : out)
+    if (r < 4 || r > 12) return false;
+  return true;
 }
 
 }  // namespace omr
 
 extern "C" omr_status omr_batch_set_max_log_radix(int log_radix) try {
-  if (log_radix < 4 || log_radix > 12) return OMR_E_SCHEDULE;
+  // below 2^5 a balanced split can leave parts of 2^3, which the batch tile
+  // kernel (register radix 16) cannot run
+  if (log_radix < 5 || log_radix > 12) return OMR_E_SCHEDULE;
   g_batch_max_log_radix = log_radix;
+  return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
+}
+
+extern "C" omr_status omr_batch_vector_schedule(uint32_t log_n, int max_log_radix, int *log_radix,
+                                                int *npass) try {
+  if (!log_radix || !npass) return OMR_E_NULL;
+  if (log_n < 4 || log_n > 36) return OMR_E_SIZE;
+  std::vector<int> sc;
+  if (!batch_schedule(log_n, max_log_radix == 0 ? g_batch_max_log_radix : max_log_radix, sc) || sc.size() > 9)
+    return OMR_E_SCHEDULE;
+  *npass = (int)sc.size();
+  for (size_t i = 0; i < sc.size(); i++) log_radix[i] = sc[i];
   return OMR_OK;
 } catch (const std::bad_alloc &) {
   return OMR_E_ALLOC;
@@ -231,9 +257,13 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
   std::vector<uint64_t> scratch_off(count, 0);
   uint32_t maxpass = 0;
   omr_status st = OMR_OK;
+  const int cap = g_batch_max_log_radix;
   for (uint32_t i = 0; i < count; i++) {
     const uint32_t logn = items[i].log_n;
-    batch_schedule(logn, p->sched[i]);
+    if (!batch_schedule(logn, cap, p->sched[i])) {
+      st = OMR_E_SCHEDULE;
+      break;
+    }
     maxpass = std::max<uint32_t>(maxpass, (uint32_t)p->sched[i].size());
     p->total_elems += 1ull << logn;
     if (p->sched[i].size() >= 2) {

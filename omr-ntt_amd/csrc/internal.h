@@ -144,6 +144,7 @@ omr_status validate_domain(int f, uint64_t n, uint64_t omega);
 omr_status ensure_buf(DevBuf &b, size_t bytes);
 void free_buf(DevBuf &b);
 omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const TwTables **out);
+void pow2_unit_of(int f, uint32_t logN, uint64_t root, TwTables &t);
 // Pins (delta +1) or unpins (-1) a cached entry for a holder that keeps its
 // device pointers past the call (batch plans bake them into device tables).
 // Call under ctx->mu.

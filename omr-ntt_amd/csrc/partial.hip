@@ -269,7 +269,12 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
     if ((st = get_tables(ctx, f, logN, omega, &tab)) != OMR_OK) return st;
     const uint32_t chunk = 1u << 14;  // CHUNK of src/encoders/dft.rs:54
     const uint64_t nchunks = (N + chunk - 1) / chunk;
-    if (nchunks == 1)
+    // One chunk writes its sums straight to d_out, unless d_out overlaps the
+    // input: every output reads all N inputs, so an in-place call must not
+    // store out[k] before the sums of out[k + 1 ..] have read x[k].
+    const char *ib = (const char *)d_in, *ob = (const char *)d_out;
+    const bool overlap = ob < ib + N * es && ib < ob + out_len * es;
+    if (nchunks == 1 && !overlap)
       return launch_dft_partial(f, d_in, d_out, N, (uint32_t)out_len, chunk, 1, tab, stream);
     if ((st = ensure_buf(ctx->part, (size_t)nchunks * out_len * es)) != OMR_OK) return st;
     if ((st = launch_dft_partial(f, d_in, ctx->part.p, N, (uint32_t)out_len, chunk, (uint32_t)nchunks,
@@ -320,11 +325,26 @@ omr_status partial_batch_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out
     return hipMemcpyAsync(d_out, d_in, batch * field_bytes(f), hipMemcpyDeviceToDevice, stream) == hipSuccess
                ? OMR_OK
                : OMR_E_DEVICE;
+  // Batches run in chunks of at most kMaxBatch vectors and 2^32 elements:
+  // a pass launch has one workgroup per tile of every vector (at most
+  // 2^28 work-items here, one per 16 elements), and the pruned full transform
+  // takes at most kMaxBatch vectors.
+  const uint64_t chunk = std::min<uint64_t>(kMaxBatch, std::max<uint64_t>(1, (1ull << 32) >> logN));
+  if (batch > chunk) {
+    const int es = field_bytes(f);
+    for (uint64_t b0 = 0; b0 < batch; b0 += chunk) {
+      const omr_status st = partial_batch_impl(ctx, f, (const char *)d_in + b0 * N * es,
+                                               (char *)d_out + b0 * out_len * es, logN,
+                                               std::min(chunk, batch - b0), omega, out_len, stream);
+      if (st != OMR_OK) return st;
+    }
+    return OMR_OK;
+  }
   if (out_len <= (1ull << kPrefixMaxLogR))
     return prefix_fused(ctx, f, d_in, N, d_out, out_len, logN, omega, out_len, prefix_log_n1(logN, out_len),
                         (uint32_t)batch, stream);
   const uint32_t l = multipass_log_n1(logN, out_len);
-  if (l != 0 && batch <= 0xFFFFFFFFull)
+  if (l != 0)
     return prefix_fused(ctx, f, d_in, N, d_out, out_len, logN, omega, out_len, l, (uint32_t)batch, stream);
   // long prefixes: the full transform of every vector, its last pass storing k < out_len
   Schedule sch;

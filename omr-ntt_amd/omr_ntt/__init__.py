@@ -114,6 +114,7 @@ def lib():
             "omr_batch_execute_grouped": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_execute_grouped_graph": (i32, [_VP, _VP, _VP, _VP]),
             "omr_batch_set_max_log_radix": (i32, [i32]),
+            "omr_batch_vector_schedule": (i32, [u32, i32, Pi, Pi]),
             "omr_num_schedule_classes": (i32, []),
             "omr_schedule_class": (i32, [i32, u64, u64, i32, Pi, Pi, Pi, Pi, ctypes.POINTER(u32)]),
             "omr_default_class": (i32, [i32, u64, u64]),
@@ -423,6 +424,15 @@ class BatchPlan:
 
 def batch_set_max_log_radix(r):
     _check(lib().omr_batch_set_max_log_radix(r))
+
+
+def batch_vector_schedule(log_n, max_log_radix=0):
+    """Pass radices (log2) the batch planner gives one vector of 2^log_n."""
+    lr = (ctypes.c_int * 9)()
+    npass = ctypes.c_int()
+    _check(lib().omr_batch_vector_schedule(log_n, max_log_radix, lr, ctypes.byref(npass)),
+           "omr_batch_vector_schedule")
+    return [lr[i] for i in range(npass.value)]
 
 
 # ---- autotuner and learned cost model ---------------------------------------

@@ -69,9 +69,15 @@ static inline uint64_t gl_reduce128(u128 x) {
   return r;
 }
 
+/* 31/32-bit fields: the remainder by a constant modulus (the compiler turns
+ * it into a multiply-shift), the same value as (a * b) % p. */
 static inline uint64_t fmul(int f, uint64_t a, uint64_t b) {
-  if (f == 2) return gl_reduce128((u128)a * b);
-  return (a * b) % ORC_P[f];
+  switch (f) {
+    case 0: return (a * b) % 2013265921ull;
+    case 1: return (a * b) % 2130706433ull;
+    case 2: return gl_reduce128((u128)a * b);
+    default: return (a * b) % 3221225473ull;
+  }
 }
 
 static uint64_t fpow(int f, uint64_t a, uint64_t e) {
@@ -109,12 +115,18 @@ uint64_t orc_root_of_unity(int f, uint64_t n) {
   return w;
 }
 
-/* src/encoder.rs:54-57 and src/encoders/utils.rs:117-119 */
+/* src/encoder.rs:54-57 and src/encoders/utils.rs:117-119:
+ * a.reverse_bits() >> (64 - log_len).  The 64-bit reversal swaps adjacent
+ * bits, pairs and nibbles, then the bytes. */
+static inline uint64_t rev64(uint64_t x) {
+  x = ((x >> 1) & 0x5555555555555555ull) | ((x & 0x5555555555555555ull) << 1);
+  x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+  x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+  return __builtin_bswap64(x);
+}
 static inline uint64_t bitrev(uint64_t a, int log_len) {
   if (log_len == 0) return 0;
-  uint64_t r = 0;
-  for (int i = 0; i < 64; i++) r |= ((a >> i) & 1ull) << (63 - i);
-  return r >> (64 - log_len);
+  return rev64(a) >> (64 - log_len);
 }
 uint64_t orc_bitrev(uint64_t a, int log_len) { return bitrev(a, log_len); }
 
@@ -135,11 +147,16 @@ void orc_domain(int f, uint64_t n, uint64_t omega, uint64_t *twiddles,
 
 /* ---- permutations and radix-2 kernels (src/encoders/utils.rs) ---------- */
 
-/* derange / derange_seq (utils.rs:85-113): swap each bit-reversed pair once. */
+/* derange (utils.rs:85-101): swap each bit-reversed pair once, in parallel
+ * over the indices like the reference's par_iter (index idx owns the pair
+ * (idx, bitrev(idx)) with idx < bitrev(idx), so no two threads touch one
+ * element).  Below 2^16 elements it runs serially, as derange_seq
+ * (utils.rs:103-113) does inside the row transforms. */
 void orc_derange(uint64_t *xi, uint64_t n, int log_len) {
   if (n < 3) return;
-  for (uint64_t idx = 1; idx < n - 1; idx++) {
-    uint64_t r = bitrev(idx, log_len);
+#pragma omp parallel for schedule(static) if (n >= (1u << 16))
+  for (int64_t i = 1; i < (int64_t)n - 1; i++) {
+    const uint64_t idx = (uint64_t)i, r = bitrev(idx, log_len);
     if (idx < r) {
       uint64_t t = xi[idx];
       xi[idx] = xi[r];
@@ -157,7 +174,8 @@ static void layer_twiddles(uint64_t n1, uint64_t n2, const uint64_t *tw, uint64_
   }
 }
 
-/* inplace_radix2_dit (utils.rs:26-55). */
+/* inplace_radix2_dit (utils.rs:26-55).  Serial: the reference runs it on
+ * each row inside a parallel loop. */
 static void radix2_dit(int f, uint64_t *xi, uint64_t n, const uint64_t *ltw) {
   if (n < 2) return;
   orc_derange(xi, n, __builtin_ctzll(n));
@@ -170,6 +188,46 @@ static void radix2_dit(int f, uint64_t *xi, uint64_t n, const uint64_t *ltw) {
         uint64_t u = lo[j], v = fmul(f, hi[j], cur[j]);
         lo[j] = fadd(f, u, v);
         hi[j] = fsub(f, u, v);
+      }
+    }
+  }
+}
+
+/* The same transform with each layer's butterflies spread over the host's
+ * threads (the ArkRadix2 role: ark-poly's radix-2 FFT with its "parallel"
+ * feature, Cargo.toml:9, splits every layer's butterflies over rayon):
+ * over the blocks while a layer has at least 64 of them, else over j
+ * inside each block.  Same butterflies, same order within each one. */
+static void radix2_dit_par(int f, uint64_t *xi, uint64_t n, const uint64_t *ltw) {
+  if (n < (1u << 12)) {
+    radix2_dit(f, xi, n, ltw);
+    return;
+  }
+  orc_derange(xi, n, __builtin_ctzll(n));
+#pragma omp parallel
+  for (uint64_t length = 2; length <= n; length *= 2) {
+    const uint64_t half = length / 2, nblk = n / length;
+    const uint64_t *cur = ltw + (half - 1);
+    if (nblk >= 64) {
+#pragma omp for schedule(static)
+      for (int64_t bb = 0; bb < (int64_t)nblk; bb++) {
+        uint64_t *lo = xi + (uint64_t)bb * length, *hi = lo + half;
+        for (uint64_t j = 0; j < half; j++) {
+          uint64_t u = lo[j], v = fmul(f, hi[j], cur[j]);
+          lo[j] = fadd(f, u, v);
+          hi[j] = fsub(f, u, v);
+        }
+      }
+    } else {
+      for (uint64_t b = 0; b < n; b += length) {
+        uint64_t *lo = xi + b, *hi = xi + b + half;
+#pragma omp for schedule(static)
+        for (int64_t jj = 0; jj < (int64_t)half; jj++) {
+          const uint64_t j = (uint64_t)jj;
+          uint64_t u = lo[j], v = fmul(f, hi[j], cur[j]);
+          lo[j] = fadd(f, u, v);
+          hi[j] = fsub(f, u, v);
+        }
       }
     }
   }
@@ -192,11 +250,21 @@ static void radix2_dif_no_derange(int f, uint64_t *xi, uint64_t n, const uint64_
   }
 }
 
-/* transpose index map (transpose_out_of_place.rs:19-29, 340-366):
- * out[y + x*H] = in[x + y*W] for x < W, y < H. */
+/* transpose_par (transpose_out_of_place.rs:19-29, 340-366):
+ * out[y + x*H] = in[x + y*W] for x < W, y < H.  Like the reference it works
+ * in 16 x 16 tiles and, for large matrices, on all host threads (the
+ * reference splits recursively with rayon::join; here the tiles are shared
+ * out by OpenMP).  Same index map, same result. */
 void orc_transpose(const uint64_t *in, uint64_t *out, uint64_t w, uint64_t h) {
-  for (uint64_t y = 0; y < h; y++)
-    for (uint64_t x = 0; x < w; x++) out[y + x * h] = in[x + y * w];
+  const uint64_t B = 16, wb = (w + B - 1) / B, hb = (h + B - 1) / B;
+#pragma omp parallel for collapse(2) schedule(static) if (w * h >= (1u << 16))
+  for (int64_t ty = 0; ty < (int64_t)hb; ty++)
+    for (int64_t tx = 0; tx < (int64_t)wb; tx++) {
+      const uint64_t y0 = (uint64_t)ty * B, x0 = (uint64_t)tx * B;
+      const uint64_t y1 = y0 + B < h ? y0 + B : h, x1 = x0 + B < w ? x0 + B : w;
+      for (uint64_t y = y0; y < y1; y++)
+        for (uint64_t x = x0; x < x1; x++) out[y + x * h] = in[x + y * w];
+    }
 }
 
 /* ---- full-transform encoders ------------------------------------------- */
@@ -207,7 +275,7 @@ void orc_radix2(int f, uint64_t *buf, uint64_t n, const uint64_t *twiddles) {
   if (n < 2) return;
   uint64_t *ltw = (uint64_t *)malloc(sizeof(uint64_t) * n);
   layer_twiddles(n, 1, twiddles, ltw);
-  radix2_dit(f, buf, n, ltw);
+  radix2_dit_par(f, buf, n, ltw);
   free(ltw);
 }
 
@@ -585,6 +653,20 @@ void orc_ntt(int f, uint64_t *buf, uint64_t n, uint64_t omega) {
   orc_domain(f, n, omega, tw, NULL);
   orc_radix2(f, buf, n, tw);
   free(tw);
+}
+
+/* A batch of vectors of uneven power-of-two sizes: vector i is the
+ * 2^log_n[i] elements at buf + off[i], forward (inv[i] == 0) or inverse
+ * transformed in place; vectors spread over the host threads. */
+void orc_ntt_batch(int f, uint64_t *buf, const uint64_t *off, const uint32_t *log_n, const uint32_t *inv,
+                   uint64_t count) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t i = 0; i < (int64_t)count; i++) {
+    const uint64_t n = 1ull << log_n[i];
+    const uint64_t w = orc_root_of_unity(f, n);
+    if (inv[i]) orc_intt(f, buf + off[i], n, w);
+    else orc_ntt(f, buf + off[i], n, w);
+  }
 }
 
 /* Coset low-degree extension: out[i] = sum_j c[j] (shift * w_M^i)^j for

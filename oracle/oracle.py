@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
+NATIVE_PATH = os.path.join(HERE, "liborc_native.so")
 
 BABYBEAR, KOALABEAR, GOLDILOCKS, TEDDYBEAR = 0, 1, 2, 3
 FIELDS = {"babybear": 0, "koalabear": 1, "goldilocks": 2, "teddybear": 3}
@@ -28,6 +29,22 @@ _U64P = ctypes.POINTER(ctypes.c_uint64)
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def use_native_build():
+    """Switches this module to the -march=native build of the oracle, compiled
+    for the host running it (the CPU baseline's build, mirroring the
+    reference's target-cpu=native, .cargo/config.toml:2).  Call before the
+    first use of lib().  Returns False (portable build kept) if the compile
+    fails."""
+    global _lib, LIB_PATH
+    if _lib is not None and LIB_PATH == NATIVE_PATH:
+        return True
+    r = subprocess.run(["make", "-s", "-B", "-C", HERE, "native"], capture_output=True, text=True)
+    if r.returncode != 0:
+        return False
+    LIB_PATH, _lib = NATIVE_PATH, None
+    return True
 
 
 def lib():
@@ -59,6 +76,7 @@ def lib():
             "orc_tfhe_stockham_radix8": (i32, [i32, vp, u64, vp]),
             "orc_transform_decomposition": (None, [i32, vp, u64, u64, vp, u64, i32]),
             "orc_intt": (None, [i32, vp, u64, u64]),
+            "orc_ntt_batch": (None, [i32, vp, vp, vp, vp, u64]),
             "orc_ntt": (None, [i32, vp, u64, u64]),
             "orc_coset_lde": (i32, [i32, vp, u64, u64, u64, vp]),
             "orc_coset_decode": (ctypes.c_int64, [i32, vp, u64, u64, u64, vp]),
@@ -189,6 +207,20 @@ def encoder(name, f, x, dom=None, s=None):
     else:
         raise KeyError(name)
     return x
+
+
+def ntt_batch(f, buf, offsets, log_ns, inverse):
+    """Vectors of uneven sizes in one buffer: vector i = buf[off[i] : off[i] +
+    2^log_ns[i]], forward (inverse[i] == 0) or inverse; returns a new array."""
+    b = _u64(buf).copy()
+    off = _u64(offsets)
+    ln = np.ascontiguousarray(np.asarray(log_ns, dtype=np.uint32))
+    iv = np.ascontiguousarray(np.asarray(inverse, dtype=np.uint32))
+    assert off.size == ln.size == iv.size
+    assert all(int(o) + (1 << int(l)) <= b.size for o, l in zip(off, ln))
+    lib().orc_ntt_batch(f, _p(b), _p(off), ln.ctypes.data_as(ctypes.c_void_p), iv.ctypes.data_as(ctypes.c_void_p),
+                        off.size)
+    return b
 
 
 def coset_lde(f, coeffs, blowup, shift=None):

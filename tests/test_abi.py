@@ -147,3 +147,22 @@ def test_missing_library_fails_loudly():
         "print('raised')\n" % os.path.join(ROOT, "omr-ntt_amd"))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "raised" in r.stdout, r.stderr
+
+
+def test_batch_vector_schedules_fit_the_tile_kernel():
+    """Every (radix cap, log n) the batch planner accepts splits log n into
+    passes of 2^4 .. 2^12 (what the batch tile kernel runs, register radix
+    16) that sum to log n; caps below 2^5 or above 2^12 are refused."""
+    for cap in range(5, 13):
+        for ln in range(4, 37):
+            sched = E.batch_vector_schedule(ln, cap)
+            assert sum(sched) == ln, (cap, ln, sched)
+            assert all(4 <= r <= 12 for r in sched), (cap, ln, sched)
+            if ln > 12:
+                assert max(sched) <= cap and max(sched) - min(sched) <= 1, (cap, ln, sched)
+    for cap in (1, 2, 3, 4, 13):
+        assert E.lib().omr_batch_set_max_log_radix(cap) == 11
+        with pytest.raises(E.OmrError):
+            E.batch_vector_schedule(16, cap)
+    with pytest.raises(E.OmrError):
+        E.batch_vector_schedule(3, 9)  # vectors start at 2^4

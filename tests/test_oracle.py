@@ -188,3 +188,22 @@ def test_large_partial_matrix():  # src/lib.rs:167-180 at sizes that run fast he
             for s in (1, 3, 101, 16383, 16384):
                 m = min(2 * s, n)
                 assert np.array_equal(O.encoder("TransformDecomposition", f, x, s=s)[:m], exp[:m])
+
+
+def test_ntt_batch_matches_per_vector_calls():
+    """orc_ntt_batch (uneven vectors on the host's threads) equals one
+    orc_ntt / orc_intt call per vector, and leaves the gaps untouched."""
+    rng = np.random.default_rng(9)
+    for f in range(4):
+        items, off = [], 0
+        for _ in range(40):
+            ln = int(rng.integers(1, 12))
+            items.append((off, ln, int(rng.integers(0, 2))))
+            off += (1 << ln) + 2
+        x = O.uniform(f, off, 3)
+        got = O.ntt_batch(f, x, [i[0] for i in items], [i[1] for i in items], [i[2] for i in items])
+        exp = x.copy()
+        for o, ln, inv in items:
+            seg = x[o:o + (1 << ln)]
+            exp[o:o + (1 << ln)] = O.intt(f, seg) if inv else O.ntt(f, seg)
+        assert np.array_equal(got, exp), f
