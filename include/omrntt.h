@@ -49,7 +49,8 @@ typedef enum {
   OMR_E_ALLOC = 9,        /* device or host allocation failed */
   OMR_E_DEVICE = 10,      /* HIP runtime or launch error */
   OMR_E_SCHEDULE = 11,    /* explicit schedule invalid for this N / field */
-  OMR_E_ARG = 12          /* other invalid argument */
+  OMR_E_ARG = 12,         /* other invalid argument */
+  OMR_E_SCRATCH = 13      /* request exceeds the fixed scratch of omr_ntt_device_noalloc */
 } omr_status;
 
 typedef enum {
@@ -114,6 +115,22 @@ omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n, 
  * here and in the other dense batched entry points (OMR_E_ARG beyond). */
 omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const void *d_in, void *d_out,
                           uint64_t n, uint64_t batch, uint64_t omega, uint64_t s, void *stream);
+
+/* No-allocation form of omr_ntt_device (full-transform variants except
+ * OMR_DFT) and omr_intt_device (inverse != 0).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc / hipFree: the twiddle tables and the
+ * intermediate vectors live in a fixed device array of the library (one per
+ * GPU, held until the process ends), the tables rebuilt on the device when
+ * (field, n, omega) changes.  n * batch * element bytes must be at most
+ * OMR_NOALLOC_MAX_BYTES (32 MiB: Goldilocks 2^22, 32-bit fields 2^23), else
+ * OMR_E_SCRATCH before any GPU work.  Calls that overlap in time on one GPU
+ * (other streams or host threads) race on that array.  Same results, bit for
+ * bit, as omr_ntt_device / omr_intt_device.  d_in may equal d_out. */
+#define OMR_NOALLOC_MAX_BYTES (32ull << 20)
+omr_status omr_ntt_device_noalloc(omr_ctx *ctx, int field, int variant, const void *d_in, void *d_out,
+                                  uint64_t n, uint64_t batch, uint64_t omega, int inverse, void *stream);
 
 /* Inverse: x[j] = n^-1 sum_k X[k] omega^(-jk).  d_in may equal d_out. */
 omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in, void *d_out, uint64_t n,

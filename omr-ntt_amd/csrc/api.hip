@@ -40,6 +40,7 @@ extern "C" const char *omr_status_str(omr_status s) {
     case OMR_E_DEVICE: return "HIP device error";
     case OMR_E_SCHEDULE: return "invalid schedule";
     case OMR_E_ARG: return "invalid argument";
+    case OMR_E_SCRATCH: return "request exceeds the no-allocation scratch (n * batch * element bytes > OMR_NOALLOC_MAX_BYTES)";
   }
   return "unknown status";
 }
@@ -412,9 +413,9 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
   if (sum != (int)logN || sch.logR.empty() || sch.logR.size() > 8) return OMR_E_SCHEDULE;
   const int np = (int)sch.logR.size();
 
-  const TwTables *tab;
-  omr_status st = get_tables(ctx, f, logN, root, &tab);
-  if (st != OMR_OK) return st;
+  const TwTables *tab = o.tables;
+  omr_status st = OMR_OK;
+  if (!tab && (st = get_tables(ctx, f, logN, root, &tab)) != OMR_OK) return st;
 
   // ping-pong buffers X_0 = in, X_np = out, X_1..X_np-1 from scratch
   std::vector<const void *> X(np + 1);
@@ -474,7 +475,7 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     if (i == np - 1) {
       // the last pass (s = 1) multiplies every input by a two-level twiddle
       // when N > 2^log_mid: the scale rides in a scaled hi table instead
-      const void *shi = (o.scale && o.scale_c && i > 0 && logN > tab->log_mid)
+      const void *shi = (o.scale && o.scale_c && !o.tables && i > 0 && logN > tab->log_mid)
                             ? scaled_hi(ctx, f, logN, root, o.scale_c)
                             : nullptr;
       if (shi) {
@@ -541,6 +542,7 @@ extern "C" omr_status omr_ctx_create(int device, omr_ctx **out) try {
     return OMR_E_DEVICE;
   }
   if (hipSetDevice(device) != hipSuccess) return OMR_E_DEVICE;
+  noalloc_init_device(device);  // once per device, with the device current
   omr_ctx *c = new (std::nothrow) omr_ctx();
   if (!c) return OMR_E_ALLOC;
   c->device = device;

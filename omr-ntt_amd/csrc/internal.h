@@ -67,6 +67,7 @@ struct XformOpts {
   const uint64_t *off_in = nullptr;   // device table: vector v's input at element off_in[v] (not in_stride)
   const uint64_t *off_out = nullptr;  // device table: vector v's output at element off_out[v] (not out_stride)
   void *scratch[2] = {nullptr, nullptr};  // caller-owned intermediates (batch x N elements each), else the context's
+  const struct TwTables *tables = nullptr;  // caller-built twiddle tables of (logN, root), else the context's cache
 };
 
 // Pinned staging ring of the host-buffer entry points (hostio.hip): kSlots
@@ -145,6 +146,9 @@ omr_status ensure_buf(DevBuf &b, size_t bytes);
 void free_buf(DevBuf &b);
 omr_status get_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, const TwTables **out);
 void pow2_unit_of(int f, uint32_t logN, uint64_t root, TwTables &t);
+// No-allocation entry point (noalloc.hip): looks up this device's static
+// scratch array once; called by omr_ctx_create with the device current.
+void noalloc_init_device(int device);
 // Pins (delta +1) or unpins (-1) a cached entry for a holder that keeps its
 // device pointers past the call (batch plans bake them into device tables).
 // Call under ctx->mu.

@@ -49,7 +49,7 @@ ARK_RADIX2, DFT, LAMBDA_BOWERS, LAMBDA_RADIX4, PLONKY3_LAYER_SPLIT, WINTERFELL_F
 
 STATUS = {0: "ok", 1: "E_NULL", 2: "E_SIZE", 3: "E_TWO_ADICITY", 4: "E_FIELD", 5: "E_VARIANT",
           6: "E_RADIX", 7: "E_OMEGA", 8: "E_OUTLEN", 9: "E_ALLOC", 10: "E_DEVICE",
-          11: "E_SCHEDULE", 12: "E_ARG"}
+          11: "E_SCHEDULE", 12: "E_ARG", 13: "E_SCRATCH"}
 
 
 class OmrError(RuntimeError):
@@ -96,6 +96,7 @@ def lib():
             "omr_ntt": (i32, [_VP, i32, i32, _VP, u64, u64, u64]),
             "omr_ntt_device": (i32, [_VP, i32, i32, _VP, _VP, u64, u64, u64, u64, _VP]),
             "omr_intt_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, _VP]),
+            "omr_ntt_device_noalloc": (i32, [_VP, i32, i32, _VP, _VP, u64, u64, u64, i32, _VP]),
             "omr_ntt_schedule_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, i32, Pi, i32, i32, _VP]),
             "omr_default_schedule": (i32, [i32, u64, Pi, Pi]),
             "omr_partial_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, i32, i32, u32, _VP]),
