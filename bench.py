@@ -31,6 +31,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def prewarm(fn, torch, seconds):
+    """Untimed calls of fn for a fixed wall time: the GPU clock ramps up over
+    ~100 ms of work after an idle period, so a short --warmup alone would time
+    the ramp.  Returns the number of calls made."""
+    calls = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(8):
+            fn(calls)
+            calls += 1
+        torch.cuda.synchronize()
+    return calls
+
+
 def time_device(fn, steps, warmup, torch):
     """Times `steps` calls of fn() with events on torch's current stream
     (the stream the engine launches on); returns seconds per call."""
@@ -47,10 +61,22 @@ def time_device(fn, steps, warmup, torch):
     return start.elapsed_time(end) / 1e3 / steps
 
 
-def cpu_baseline(x, n, budget_s=20.0):
-    """WinterfellFourStep restated in C (oracle/), OpenMP over host cores."""
+def _oracle_native():
+    """The oracle module on its -march=native build, compiled on this host
+    (the reference builds with target-cpu=native, .cargo/config.toml:2);
+    returns (module, build description)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    native = O.use_native_build()
+    return O, ("gcc -O3 -march=native -fopenmp (built on this host)" if native
+               else "gcc -O3 -march=x86-64-v3 -fopenmp (native build failed)")
+
+
+def cpu_baseline(x, n, budget_s=20.0):
+    """WinterfellFourStep restated in C (oracle/): transposes tiled and
+    spread over the host's threads like transpose_par, rows in parallel,
+    derange in parallel (winterfell_four_step.rs:31-81, utils.rs:85-101)."""
+    O, build = _oracle_native()
     f = O.GOLDILOCKS
     d = O.Domain(f, n)
     times = []
@@ -64,7 +90,7 @@ def cpu_baseline(x, n, budget_s=20.0):
     med = times[len(times) // 2]
     return {"value": n / med, "unit": "elems/s", "cores": O.num_threads(), "kind": "port",
             "sample": f"WinterfellFourStep (C restatement) Goldilocks 2^{n.bit_length() - 1}, "
-                      f"median of {len(times)} runs"}, out
+                      f"median of {len(times)} runs", "build": build}, out
 
 
 def cpu_extras(np, budget_s=4.0):
@@ -73,11 +99,11 @@ def cpu_extras(np, budget_s=4.0):
     encoders that run them, OpenMP over the host cores, median of up
     to 10 runs within a time budget per item (inputs built outside the
     timing, like criterion's iter_batched)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+    O, _ = _oracle_native()
     items = [
         ("babybear_2p16_radix2_dit", O.BABYBEAR, 16, "ArkRadix2", None),
         ("goldilocks_2p20_winterfell", O.GOLDILOCKS, 20, "WinterfellFourStep", None),
+        ("goldilocks_2p24_winterfell", O.GOLDILOCKS, 24, "WinterfellFourStep", None),
         ("goldilocks_2p22_winterfell", O.GOLDILOCKS, 22, "WinterfellFourStep", None),
         ("goldilocks_2p24_stockham8", O.GOLDILOCKS, 24, "TfheStockhamRadix8", None),
         ("goldilocks_2p24_plonky3", O.GOLDILOCKS, 24, "Plonky3Radix2LayerSplit", None),
@@ -105,18 +131,42 @@ def cpu_extras(np, budget_s=4.0):
 
 
 def other_sizes(E, F, torch, np):
+    """Goldilocks 2^20 / 2^22 forward and inverse (default schedules), inputs
+    rotating over buffer sets larger than the 256 MB Infinity Cache; GB/s =
+    16 N algorithmic bytes (read + write N words) per transform."""
     out = {}
     for ln in (20, 22):
         n = 1 << ln
-        nb = max(3, (1 << 26) // n)  # rotate past the Infinity Cache
+        nb = max(3, (1 << 26) // n)  # 2 x nb x 8 n bytes >= 1 GiB
         xs = [E.to_device(np.random.default_rng(ln + i).integers(0, F.p, size=n, dtype=np.uint64), F)
               for i in range(nb)]
         ys = [E.empty_device(n, F) for _ in range(nb)]
         fwd = time_device(lambda i: E.ntt_device(xs[i % nb], ys[i % nb], F), 200, 200, torch)
         inv = time_device(lambda i: E.intt_device(ys[i % nb], xs[i % nb], F), 200, 200, torch)
+        gbs = 16 * n / fwd / 1e9
         out[f"2^{ln}"] = {"forward_us": fwd * 1e6, "inverse_us": inv * 1e6, "schedule": E.default_schedule(F, n),
-                          "elems_per_s": n / fwd}
-        log(f"bench: 2^{ln} forward {fwd * 1e6:.1f} us inverse {inv * 1e6:.1f} us")
+                          "elems_per_s": n / fwd, "alg_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        log(f"bench: 2^{ln} forward {fwd * 1e6:.1f} us ({gbs:.0f} GB/s, {gbs / HBM_PEAK_GBS:.1%} of peak) "
+            f"inverse {inv * 1e6:.1f} us")
+    return out
+
+
+def variants_2p24(E, F, torch, np):
+    """Goldilocks 2^24 under the reference variants that name a structure:
+    WinterfellFourStep (n1 = n2 = 2^12: two passes of radix 4096) and
+    TfheStockhamRadix8 (radix-8 digits), three rotating HBM-resident inputs."""
+    n = 1 << 24
+    xs = [E.to_device(np.random.default_rng(240 + i).integers(0, F.p, size=n, dtype=np.uint64), F)
+          for i in range(3)]
+    ys = [E.empty_device(n, F) for _ in range(3)]
+    out = {}
+    for name, v in (("WinterfellFourStep", E.WINTERFELL_FOUR_STEP), ("TfheStockhamRadix8", E.TFHE_STOCKHAM_RADIX8)):
+        t = time_device(lambda i: E.ntt_device(xs[i % 3], ys[i % 3], F, v), 100, 50, torch)
+        gbs = 16 * n / t / 1e9
+        out[name] = {"forward_us": t * 1e6, "elems_per_s": n / t, "alg_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        log(f"bench: Goldilocks 2^24 {name} {t * 1e6:.1f} us ({gbs:.0f} GB/s)")
+    del xs, ys
+    torch.cuda.empty_cache()
     return out
 
 
@@ -181,23 +231,29 @@ def small_babybear(E, torch, np):
 
 
 def partial_2p22(E, torch, np):
-    """Partial outputs of N = 2^22: prefix (out_len) and stride subset."""
+    """Partial outputs of N = 2^22: prefix (out_len) and stride subset.
+    Inputs rotate over 32 vectors (512 MiB BabyBear, 1 GiB Goldilocks), more
+    than the 256 MB Infinity Cache holds, so each call reads from HBM."""
     out = {}
     n = 1 << 22
+    nb = 32
     for F in (E.BabyBear, E.Goldilocks):
-        x = E.to_device(np.random.default_rng(22).integers(0, F.p, size=n, dtype=np.uint64), F)
+        base = E.to_device(np.random.default_rng(22).integers(0, F.p, size=n, dtype=np.uint64), F)
+        xs = [base] + [torch.roll(base, k) for k in range(1, nb)]
         y = E.empty_device(n, F)
-        full = time_device(lambda i: E.ntt_device(x, y, F), 100, 100, torch)
-        rows = {"full_us": full * 1e6}
+        full = time_device(lambda i: E.ntt_device(xs[i % nb], y, F), 100, 100, torch)
+        rows = {"full_us": full * 1e6, "rotating_inputs": nb, "rotating_bytes": nb * n * F.bytes}
         for lo in (8, 12, 16, 20):
             m = 1 << lo
             o = E.empty_device(m, F)
-            pre = time_device(lambda i: E.partial_device(x, o, F, m, mode=E.PREFIX), 100, 100, torch)
-            strd = time_device(lambda i: E.partial_device(x, o, F, m, mode=E.STRIDE), 100, 100, torch)
-            rows[f"2^{lo}"] = {"prefix_us": pre * 1e6, "stride_us": strd * 1e6}
+            pre = time_device(lambda i: E.partial_device(xs[i % nb], o, F, m, mode=E.PREFIX), 100, 100, torch)
+            strd = time_device(lambda i: E.partial_device(xs[i % nb], o, F, m, mode=E.STRIDE), 100, 100, torch)
+            rows[f"2^{lo}"] = {"prefix_us": pre * 1e6, "stride_us": strd * 1e6,
+                               "prefix_in_GBps": n * F.bytes / pre / 1e9}
         out[F.name] = rows
+        del xs
         log(f"bench: partial {F.name} 2^22: " + ", ".join(
-            f"{k} {v['prefix_us']:.1f}/{v['stride_us']:.1f} us" for k, v in rows.items() if k != "full_us")
+            f"{k} {v['prefix_us']:.1f}/{v['stride_us']:.1f} us" for k, v in rows.items() if isinstance(v, dict))
             + f" (full {full * 1e6:.1f} us)")
     return out
 
@@ -434,6 +490,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--logn", type=int, default=24)
+    ap.add_argument("--prewarm-s", type=float, default=0.5,
+                    help="untimed transforms for this many seconds before --warmup (clock ramp-up)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 128 MiB device copy (torch copy_) 10 times, to calibrate "
@@ -502,6 +560,7 @@ def main():
     def fwd(i):
         E.ntt_device(xs[i % nbuf], ys[i % nbuf], F)
 
+    prewarm_calls = prewarm(fwd, torch, args.prewarm_s)
     sec = time_device(fwd, args.steps, args.warmup, torch)
     log(f"bench: forward {sec * 1e6:.1f} us/transform")
 
@@ -517,6 +576,7 @@ def main():
         E.intt_device(ys[i % nbuf], xs[i % nbuf], F)
 
     # intt(ntt(x)) == x, so the inverse steps rewrite the inputs unchanged
+    prewarm(inv, torch, min(args.prewarm_s, 0.2))
     sec_inv = time_device(inv, args.steps, args.warmup, torch)
     log(f"bench: inverse {sec_inv * 1e6:.1f} us/transform, round trip ok={round_trip}")
 
@@ -541,6 +601,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        # untimed transforms before the --warmup ones, for a fixed wall time
+        "prewarm": {"seconds": args.prewarm_s, "transforms": prewarm_calls},
         "ms_per_step": sec * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -570,6 +632,7 @@ def main():
         result["host_path"] = host_path(E, F, n, host_x, np)
     if "sizes" in legs:
         result["sizes_us"] = other_sizes(E, F, torch, np)
+        result["variants_2p24"] = variants_2p24(E, F, torch, np)
     if "fields" in legs:
         result["fields_2p24"] = other_fields(E, torch, np)
     if "batched" in legs:

@@ -475,7 +475,10 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     if (i == np - 1) {
       // the last pass (s = 1) multiplies every input by a two-level twiddle
       // when N > 2^log_mid: the scale rides in a scaled hi table instead
-      const void *shi = (o.scale && o.scale_c && !o.tables && i > 0 && logN > tab->log_mid)
+      // (not for a radix-4096 Goldilocks pass: it takes the scale in the
+      // start value of its twiddle chains)
+      const bool fs = fs4096_ok(f, logR, tab->unit(), logN, logN - consumed, kEpiScale);
+      const void *shi = (o.scale && o.scale_c && !o.tables && !fs && i > 0 && logN > tab->log_mid)
                             ? scaled_hi(ctx, f, logN, root, o.scale_c)
                             : nullptr;
       if (shi) {

@@ -55,6 +55,13 @@ inline int pass_epilogue(const PassArgs &a) {
   return (a.flags & kScale) ? kEpiScale : kEpiPlain;
 }
 
+// Goldilocks radix-4096 passes as two 64-point register DFTs (fourstep.h):
+// whether a pass qualifies (forward / inverse ark roots, 2^15 <= N <= 2^29,
+// 8 columns of one p per tile, plain or scaled epilogue), and its launch
+// (total_tiles = 8-column tiles of all vectors, 512 threads each).
+bool fs4096_ok(int field, int logR, int unit, uint32_t logN, uint32_t logS, int epi);
+hipError_t launch_pass_fs4096(int unit, int epi, const PassArgs &a, uint64_t total_tiles, hipStream_t stream);
+
 // Per-field entry points (one translation unit each).
 hipError_t launch_pass_babybear(int logR, int logE, int epi, const PassArgs &a, uint64_t t, dim3 b,
                                 hipStream_t s);
