@@ -26,8 +26,13 @@ FLAGS = [
 ]
 
 
-def _compile(src):
+def _compile(src, incremental=False):
     obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
+    if incremental and os.path.exists(obj):
+        deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".h", ".inc"))]
+        deps.append(os.path.join(os.path.dirname(HERE), "include", "omrntt.h"))
+        if all(os.path.getmtime(d) <= os.path.getmtime(obj) for d in deps):
+            return obj
     cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     errs = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
@@ -36,12 +41,14 @@ def _compile(src):
     return obj
 
 
-def build(jobs=None, verbose=True):
+def build(jobs=None, verbose=True, incremental=False):
+    """Compiles every source (incremental=True: only those older than a
+    source or header they may include) and links the library."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda src: _compile(src, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -60,9 +67,11 @@ def clean():
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only objects older than a source or header")
     a = ap.parse_args()
     try:
-        build(a.jobs)
+        build(a.jobs, incremental=a.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

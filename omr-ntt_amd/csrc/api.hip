@@ -162,44 +162,8 @@ void pin_tables(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, int delta) {
   else if (it->second.pins) it->second.pins--;
 }
 
-// hi table of the entry (f, logN, root) times the canonical scale sc, built
-// on first use and kept with the entry (rebuilt if the scale changes).
-static const void *scaled_hi(omr_ctx *ctx, int f, uint32_t logN, uint64_t root, uint64_t sc) {
-  auto it = ctx->tables.find(std::make_tuple(f, logN, root));
-  if (it == ctx->tables.end()) return nullptr;
-  TwTables &t = it->second;
-  if (t.hi_scaled && t.hi_scale == sc) return t.hi_scaled;
-  const int es = field_bytes(f);
-  const uint64_t nhi = 1ull << (logN - t.lo_bits);
-  const uint64_t step = fpow_host(f, root, 1ull << t.lo_bits);
-  std::vector<uint64_t> v(nhi);
-  std::vector<uint32_t> v32;
-  uint64_t cur = sc % kP[f];
-  for (uint64_t j = 0; j < nhi; j++) {
-    v[j] = to_w_host(f, cur);
-    cur = fmul_host(f, cur, step);
-  }
-  const void *src = v.data();
-  if (es == 4) {
-    v32.assign(v.begin(), v.end());
-    src = v32.data();
-  }
-  if (!t.hi_scaled) {
-    if (hipMalloc(&t.hi_scaled, nhi * es) != hipSuccess) {
-      (void)hipGetLastError();
-      t.hi_scaled = nullptr;
-      return nullptr;
-    }
-  } else if (hipDeviceSynchronize() != hipSuccess) {  // the old scale may still be read
-    return nullptr;
-  }
-  if (hipMemcpy(t.hi_scaled, src, nhi * es, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-  t.hi_scale = sc;
-  return t.hi_scaled;
-}
-
 static void free_tables(TwTables &t) {
-  for (void **p : {&t.lo, &t.hi, &t.root, &t.pyr, &t.mid, &t.hi_scaled})
+  for (void **p : {&t.lo, &t.hi, &t.root, &t.pyr, &t.mid})
     if (*p) (void)hipFree(*p), *p = nullptr;
 }
 
@@ -473,17 +437,10 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.flags = 0;
     if (i > 0) a.flags |= kTwiddle;
     if (i == np - 1) {
-      // the last pass (s = 1) multiplies every input by a two-level twiddle
-      // when N > 2^log_mid: the scale rides in a scaled hi table instead
-      // (not for a radix-4096 Goldilocks pass: it takes the scale in the
-      // start value of its twiddle chains)
-      const bool fs = fs4096_ok(f, logR, tab->unit(), logN, logN - consumed, kEpiScale);
-      const void *shi = (o.scale && o.scale_c && !o.tables && !fs && i > 0 && logN > tab->log_mid)
-                            ? scaled_hi(ctx, f, logN, root, o.scale_c)
-                            : nullptr;
-      if (shi) {
-        a.tw_hi = shi;
-      } else if (o.scale) {
+      // a last pass with an inter-pass twiddle takes the scale in the start
+      // value of its twiddle chains (one product per thread), otherwise
+      // every output is scaled
+      if (o.scale) {
         a.flags |= kScale;
         a.scale_w = o.scale_w;
       }
@@ -762,8 +719,7 @@ extern "C" omr_status omr_intt_device(omr_ctx *ctx, int field, const void *d_in,
   if (!tuned_schedule_for(field, logN, batch, s)) default_schedule(field, logN, s);
   XformOpts o;
   o.scale = true;
-  o.scale_c = finv_host(field, n % kP[field]);
-  o.scale_w = to_w_host(field, o.scale_c);
+  o.scale_w = to_w_host(field, finv_host(field, n % kP[field]));
   return run_transform(ctx, field, d_in, d_out, logN, finv_host(field, omega), s, batch, n, n, o,
                        (hipStream_t)stream);
 } catch (const std::bad_alloc &) {
@@ -803,8 +759,7 @@ extern "C" omr_status omr_ntt_schedule_device(omr_ctx *ctx, int field, const voi
   if (inverse) {
     root = finv_host(field, omega);
     o.scale = true;
-    o.scale_c = finv_host(field, n % kP[field]);
-    o.scale_w = to_w_host(field, o.scale_c);
+    o.scale_w = to_w_host(field, finv_host(field, n % kP[field]));
   }
   return run_transform(ctx, field, d_in, d_out, logN, root, s, batch, n, n, o, (hipStream_t)stream);
 } catch (const std::bad_alloc &) {

@@ -560,8 +560,7 @@ extern "C" omr_status omr_batch_execute_plain(omr_batch_plan *p, const void *d_i
     if (it.inverse) {
       w = finv_host(p->field, w);
       o.scale = true;
-      o.scale_c = finv_host(p->field, (1ull << it.log_n) % field_modulus(p->field));
-      o.scale_w = to_w_host(p->field, o.scale_c);
+      o.scale_w = to_w_host(p->field, finv_host(p->field, (1ull << it.log_n) % field_modulus(p->field)));
     }
     const uint64_t n = 1ull << it.log_n;
     omr_status st = run_transform(ctx, p->field, (const char *)d_in + it.in_offset * es,
@@ -603,8 +602,7 @@ static omr_status grouped_launches(omr_batch_plan *p, const void *d_in, void *d_
     }
     if (g.inverse) {
       o.scale = true;
-      o.scale_c = finv_host(p->field, (1ull << g.log_n) % field_modulus(p->field));
-      o.scale_w = to_w_host(p->field, o.scale_c);
+      o.scale_w = to_w_host(p->field, finv_host(p->field, (1ull << g.log_n) % field_modulus(p->field)));
     }
     st = run_transform(p->ctx, p->field, d_in, d_out, g.log_n, g.root, g.sch, g.count, 0, 0, o,
                        fork ? p->lane_stream[g.lane] : s);

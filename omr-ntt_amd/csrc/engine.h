@@ -323,9 +323,43 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
       }
     }
 
+    bool scaled_in = false;  // kEpiScale folded into the twiddle chains
     if (a.flags & kTwiddle) {  // w_N^(k p s) = w_(N/s)^(k p) on input row k
       const uint32_t logNs = a.logN - logS;
-      if (logNs <= a.log_mid) {  // one lookup: w_(N/s)^e from the root pyramid or the mid table
+      const bool one = logNs <= a.log_mid;
+      // Rows k = t + m RE: w^(k p) = A B^m with A = w^(t p), B = w^(RE p), in
+      // two chains (even / odd m) stepped by B^2: two table lookups per
+      // thread instead of one or two per element.  Used where it saves loads
+      // at no extra products (the two-level case), and for 32-bit fields
+      // everywhere (their products cost 6 VALU slots, a lookup more).
+      if (sizeof(T) == 4 || !one) {
+        const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
+        const bool small = logNs <= a.log_rmax;
+        const T *tab = small ? pyr + (1u << logNs) : (const T *)a.tw_mid;
+        const uint32_t sh = small ? 0u : a.log_mid - logNs;
+        const uint32_t lomask = (1u << a.lo_bits) - 1;
+        auto tw = [&](uint32_t e) -> T {  // w_(N/s)^e, e < N/s
+          if (one) return tab[e << sh];
+          const uint32_t en = e << logS;  // as a power of w_N
+          return F::wmul(tlo[en & lomask], thi[en >> a.lo_bits]);
+        };
+        T A = tw(t * p);
+        if (EPI == kEpiScale) {  // the inverse's scale rides in A: once per element
+          A = F::wmul(A, (T)a.scale_w);
+          scaled_in = true;
+        }
+        const T B = tw((uint32_t)RE * p);
+        const T B2 = F::wmul(B, B);
+        T ch[2] = {A, F::wmul(A, B)};
+#pragma unroll
+        for (int m = 0; m < E; m += 2) {
+#pragma unroll
+          for (int r = 0; r < 2; r++) {
+            x[m + r] = F::mulw(x[m + r], ch[r]);
+            if (m + 2 < E) ch[r] = F::wmul(ch[r], B2);
+          }
+        }
+      } else if (one) {  // one lookup: w_(N/s)^e from the root pyramid or the mid table
         const bool small = logNs <= a.log_rmax;
         const T *tab = small ? pyr + (1u << logNs) : (const T *)a.tw_mid;
         const uint32_t sh = small ? 0u : a.log_mid - logNs;
@@ -407,7 +441,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
       return;
     }
     const uint32_t logNR = a.logN - LOGR;
-    const bool scale = EPI == kEpiScale || (EPI == kEpiGeneral && (a.flags & kScale) != 0);
+    const bool scale = (EPI == kEpiScale && !scaled_in) || (EPI == kEpiGeneral && (a.flags & kScale) != 0);
     const bool pruned = EPI == kEpiGeneral && a.store_limit < (1ull << a.logN);
     const T sw = (T)a.scale_w;
     uint32_t nonzero = 0;

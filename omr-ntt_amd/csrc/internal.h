@@ -37,8 +37,6 @@ struct TwTables {
   uint32_t pow2_unit = 0;  // Goldilocks: 2^pow2_unit = root^(2^logN / M), M = 2^log_m = min(2^logN, 64)
   uint32_t log_m = 0;
   uint32_t pins = 0;  // holders beyond one call (batch plans): the cache never frees a pinned entry
-  void *hi_scaled = nullptr;  // hi times hi_scale (inverse transforms fold n^-1 into the last twiddle)
-  uint64_t hi_scale = 0;      // canonical
   // register-DFT constant unit of the pass kernels (0 = table constants)
   int unit() const { return (log_m == 6 && (pow2_unit == 39 || pow2_unit == 153)) ? (int)pow2_unit : 0; }
 };
@@ -54,8 +52,6 @@ struct Schedule {
 struct XformOpts {
   bool scale = false;           // multiply every output by scale_w (last pass)
   uint64_t scale_w = 0;         // multiplier form
-  uint64_t scale_c = 0;         // the same scale, canonical: if set, a last pass with two-level
-                                // twiddles takes it in its hi table instead of a product per output
   bool zero_tail = false;       // input positions >= 2^log_valid read as zero (pass 0)
   uint32_t log_valid = 0;
   const void *pre_lo = nullptr;  // input-position multiplier tables (pass 0), or null

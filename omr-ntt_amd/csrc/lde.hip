@@ -111,8 +111,7 @@ extern "C" omr_status omr_coset_decode_device(omr_ctx *ctx, int field, const voi
   if (!tuned_schedule_for(field, logM, ncols, s)) default_schedule(field, logM, s);
   XformOpts o;
   o.scale = true;
-  o.scale_c = finv_host(field, M % field_modulus(field));
-  o.scale_w = to_w_host(field, o.scale_c);
+  o.scale_w = to_w_host(field, finv_host(field, M % field_modulus(field)));
   if (sinv != 1) {
     o.post_lo = sh->lo;
     o.post_hi = sh->hi;

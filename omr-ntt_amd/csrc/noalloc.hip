@@ -184,7 +184,6 @@ extern "C" omr_status omr_ntt_device_noalloc(omr_ctx *ctx, int field, int varian
     root = finv_host(field, omega);
     o.scale = true;
     o.scale_w = to_w_host(field, finv_host(field, n % field_modulus(field)));
-    // o.scale_c stays 0: the scaled hi table would be an allocation
   }
   hipStream_t hs = (hipStream_t)stream;
   TwTables t;
