@@ -107,6 +107,14 @@ int omr_variant_supports(int variant, uint32_t log_n);
  * dft.rs:88).  Synchronous. */
 omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n, uint64_t omega,
                    uint64_t s);
+/* The same on a buffer of `word_bytes`-wide words: 8 for BabyBear /
+ * KoalaBear / TeddyBear held one value per 64-bit word (the one-u64-limb
+ * storage of the crate's define_field! types, src/fields.rs:1-17, so a Rust
+ * `&mut [F]` is handed over as is; each word < p with a zero upper half;
+ * ark-ff's Montgomery words pass through because every variant is linear).
+ * 0 or the field's width is omr_ntt.  OMR_E_ARG for other widths. */
+omr_status omr_ntt_words(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n, uint64_t omega,
+                         uint64_t s, uint32_t word_bytes);
 
 /* ---- device-resident entry points --------------------------------------- */
 /* Forward transform of `batch` vectors of n elements (consecutive in memory,
@@ -115,6 +123,14 @@ omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n, 
  * here and in the other dense batched entry points (OMR_E_ARG beyond). */
 omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const void *d_in, void *d_out,
                           uint64_t n, uint64_t batch, uint64_t omega, uint64_t s, void *stream);
+/* omr_ntt_device on vectors of `word_bytes`-wide words (see omr_ntt_words;
+ * vector b at element b*n of 64-bit words).  Full variants other than Dft
+ * read 64-bit words in their first pass and write them in their last pass
+ * (no extra pass over memory); Dft and the partial variants run on a
+ * narrowed copy and widen the out_len results. */
+omr_status omr_ntt_device_words(omr_ctx *ctx, int field, int variant, const void *d_in, void *d_out,
+                                uint64_t n, uint64_t batch, uint64_t omega, uint64_t s, uint32_t word_bytes,
+                                void *stream);
 
 /* No-allocation form of omr_ntt_device (full-transform variants except
  * OMR_DFT) and omr_intt_device (inverse != 0).  No counterpart in the

@@ -448,6 +448,8 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
       a.post_hi = o.post_hi;
       a.nonzero_count = o.nonzero_count;
     }
+    if (i == 0 && o.wide_in) a.flags |= kWideIn;
+    if (i == np - 1 && o.wide_out) a.flags |= kWideOut;
     if (i == 0) {
       if (o.zero_tail) {
         a.flags |= kZeroTail;
@@ -525,6 +527,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) try {
   free_buf(ctx->part);
   free_buf(ctx->part2);
   for (auto &b : ctx->hio) free_buf(b);
+  for (auto &b : ctx->words) free_buf(b);
   free_stage(ctx->stage);
   delete ctx;
   return OMR_OK;
@@ -638,7 +641,7 @@ namespace omr {
 
 // Full forward transform of `batch` dense vectors under `variant`.
 omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void *d_out,
-                        uint32_t logN, uint64_t batch, uint64_t omega, hipStream_t stream) {
+                        uint32_t logN, uint64_t batch, uint64_t omega, hipStream_t stream, bool wide) {
   const int es = field_bytes(f);
   const uint64_t N = 1ull << logN;
   if (variant == OMR_DFT) {
@@ -667,6 +670,7 @@ omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void
   if (variant == OMR_ARK_RADIX2 && !tuned_schedule_for(f, logN, batch, s) && batched_slot(f, logN, batch))
     default_schedule(f, logN, s);
   XformOpts o;
+  o.wide_in = o.wide_out = wide;
   return run_transform(ctx, f, d_in, d_out, logN, omega, s, batch, N, N, o, stream);
 }
 
@@ -697,6 +701,65 @@ extern "C" omr_status omr_ntt_device(omr_ctx *ctx, int field, int variant, const
     char *out = (char *)d_out + b * n * es;
     st = partial_impl(ctx, field, in, out, logN, omega, out_len, 0, method, log_n1, hs);
     if (st != OMR_OK) return st;
+  }
+  return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
+}
+
+// NttEncoder::ntt on buffers of 64-bit words for a 31/32-bit field (the
+// one-limb layout of src/fields.rs's define_field! types): full variants of
+// the pass engine narrow in their first pass's loads and widen in their last
+// pass's stores; Dft and the partial variants run on a narrowed copy.
+extern "C" omr_status omr_ntt_device_words(omr_ctx *ctx, int field, int variant, const void *d_in, void *d_out,
+                                           uint64_t n, uint64_t batch, uint64_t omega, uint64_t s,
+                                           uint32_t word_bytes, void *stream) try {
+  if (!field_ok(field)) return OMR_E_FIELD;
+  const int es = field_bytes(field);
+  if (word_bytes == 0 || word_bytes == (uint32_t)es)
+    return omr_ntt_device(ctx, field, variant, d_in, d_out, n, batch, omega, s, stream);
+  if (word_bytes != 8) return OMR_E_ARG;  // 32-bit fields in 64-bit words only
+  if (variant < 0 || variant >= OMR_VARIANT_COUNT) return OMR_E_VARIANT;
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  const uint32_t logN = ilog2(n);
+  if (!omr_variant_supports(variant, logN)) return OMR_E_RADIX;
+  if (batch > kMaxBatch) return OMR_E_ARG;
+  if (!ctx || !d_in || !d_out) return OMR_E_NULL;
+  if (batch == 0) return OMR_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  hipStream_t hs = (hipStream_t)stream;
+  StreamOrder ord(ctx, hs);
+  const uint64_t *in64 = (const uint64_t *)d_in;
+  uint64_t *out64 = (uint64_t *)d_out;
+  if (n == 1)  // the size-1 transform is the identity
+    return d_in == d_out || hipMemcpyAsync(d_out, d_in, batch * 8, hipMemcpyDeviceToDevice, hs) == hipSuccess
+               ? OMR_OK
+               : OMR_E_DEVICE;
+  if (!is_partial_variant(variant) && variant != OMR_DFT)
+    return forward_full(ctx, field, variant, d_in, d_out, logN, batch, omega, hs, /*wide=*/true);
+  // narrowed copy in, the 32-bit path, widened copy of what it wrote
+  if ((st = ensure_buf(ctx->words[0], (size_t)n * batch * 4)) != OMR_OK) return st;
+  if ((st = launch_narrow(in64, (uint32_t *)ctx->words[0].p, n * batch, hs)) != OMR_OK) return st;
+  const uint32_t *x32 = (const uint32_t *)ctx->words[0].p;
+  if (variant == OMR_DFT) {
+    if ((st = ensure_buf(ctx->words[1], (size_t)n * batch * 4)) != OMR_OK) return st;
+    if ((st = forward_full(ctx, field, variant, x32, ctx->words[1].p, logN, batch, omega, hs)) != OMR_OK) return st;
+    return launch_widen((const uint32_t *)ctx->words[1].p, out64, n * batch, hs);
+  }
+  const uint64_t out_len = partial_out_len(s, n);
+  if (out_len == 0) return OMR_OK;
+  const int method = variant == OMR_DFT_PARTIAL ? 1 : 2;
+  const uint32_t log_n1 = method == 2 ? td_log_n1(es, logN, out_len) : 0;
+  if ((st = ensure_buf(ctx->words[1], (size_t)out_len * 4)) != OMR_OK) return st;
+  for (uint64_t b = 0; b < batch; b++) {
+    if ((st = partial_impl(ctx, field, x32 + b * n, ctx->words[1].p, logN, omega, out_len, 0, method, log_n1, hs)) !=
+        OMR_OK)
+      return st;
+    if ((st = launch_widen((const uint32_t *)ctx->words[1].p, out64 + b * n, out_len, hs)) != OMR_OK) return st;
   }
   return OMR_OK;
 } catch (const std::bad_alloc &) {
@@ -853,6 +916,38 @@ extern "C" omr_status omr_ntt(omr_ctx *ctx, int field, int variant, void *buf, u
   }
   if (st != OMR_OK) return st;
   return device_to_host(ctx, buf, result, (size_t)copy_back * es);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
+}
+
+// NttEncoder::ntt on a host buffer of word_bytes-wide words (64-bit words for
+// a 31/32-bit field: the crate's own `[F]` storage, one u64 limb each, whose
+// Montgomery form passes through unchanged): copy in, omr_ntt_device_words,
+// copy out.  word_bytes 0 or the field's width is omr_ntt.
+extern "C" omr_status omr_ntt_words(omr_ctx *ctx, int field, int variant, void *buf, uint64_t n, uint64_t omega,
+                                    uint64_t s, uint32_t word_bytes) try {
+  if (!field_ok(field)) return OMR_E_FIELD;
+  const int es = field_bytes(field);
+  if (word_bytes == 0 || word_bytes == (uint32_t)es) return omr_ntt(ctx, field, variant, buf, n, omega, s);
+  if (word_bytes != 8) return OMR_E_ARG;
+  if (variant < 0 || variant >= OMR_VARIANT_COUNT) return OMR_E_VARIANT;
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (!omr_variant_supports(variant, ilog2(n))) return OMR_E_RADIX;
+  if (!ctx || !buf) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->host_mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const size_t bytes = (size_t)n * 8;
+  const uint64_t back = is_partial_variant(variant) ? partial_out_len(s, n) : n;
+  if (back == 0) return OMR_OK;
+  if ((st = ensure_buf(ctx->hio[0], bytes)) != OMR_OK) return st;
+  if ((st = host_to_device(ctx, ctx->hio[0].p, buf, bytes)) != OMR_OK) return st;
+  if ((st = omr_ntt_device_words(ctx, field, variant, ctx->hio[0].p, ctx->hio[0].p, n, 1, omega, s, 8, nullptr)) !=
+      OMR_OK)
+    return st;
+  return device_to_host(ctx, buf, ctx->hio[0].p, (size_t)back * 8);
 } catch (const std::bad_alloc &) {
   return OMR_E_ALLOC;
 } catch (...) {

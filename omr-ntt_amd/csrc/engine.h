@@ -15,6 +15,8 @@
 // through LDS).  Inside the tile the R-point DFT is itself a Stockham sequence
 // of register radix-E DFTs with LDS exchanges between them.
 #pragma once
+#include <type_traits>
+
 #include "field.h"
 
 namespace omr {
@@ -44,7 +46,10 @@ struct PassArgs {
   uint32_t log_mid;
   const void *root_pyr;  // root pyramid: w_(2^L)^m at [2^L + m], m < 2^L, L <= log_rmax (multiplier form)
 };
-enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u };
+// kWideIn / kWideOut (31/32-bit fields): the pass reads / writes its vectors
+// as 64-bit words, one canonical value in the low half of each (a one-limb
+// big-integer layout), narrowing on load and widening on store.
+enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u, kWideIn = 16u, kWideOut = 32u };
 
 // Pass-kernel variants by the options they honour (chosen per launch): the
 // options' per-element tests, if compiled in, cost VALU and split the store
@@ -220,8 +225,9 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 
 template <class F, int LOGR, int LOGE, int EPI>
 __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const typename F::T *__restrict__ src,
-                                          const PassArgs &a, uint32_t tile, uint32_t tid, uint32_t c,
-                                          uint32_t t, bool staged, uint32_t logC) {
+                                          const uint64_t *__restrict__ srcw, bool wide, const PassArgs &a,
+                                          uint32_t tile, uint32_t tid, uint32_t c, uint32_t t, bool staged,
+                                          uint32_t logC) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   const uint32_t logS = a.logS;
@@ -233,6 +239,22 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
   } else {
     pos0 = (tile << (logC + LOGR)) + tid;
     step = (1u << logC) * K::RE;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (wide) {  // 64-bit words: the value is the low half
+      if (EPI == kEpiGeneral && (a.flags & kZeroTail)) {
+        const uint64_t valid = 1ull << a.logValid;
+#pragma unroll
+        for (int m = 0; m < K::E; m++) {
+          const uint32_t pos = pos0 + (uint32_t)m * step;
+          xn[m] = ((uint64_t)pos < valid) ? (T)ld_stream(srcw + pos) : (T)0;
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < K::E; m++) xn[m] = (T)ld_stream(srcw + pos0 + (uint32_t)m * step);
+      }
+      return;
+    }
   }
   if (EPI == kEpiGeneral && (a.flags & kZeroTail)) {
     const uint64_t valid = 1ull << a.logValid;
@@ -281,8 +303,13 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     const uint64_t gt = blockIdx.x;  // grid = nvec << logTiles tiles
     const uint64_t vec = gt >> logTiles;
     const uint32_t tile = (uint32_t)(gt & ((1ull << logTiles) - 1));
-    const T *__restrict__ src = (const T *)a.src + (a.vec_off_in ? a.vec_off_in[vec] : vec * a.vec_stride_in);
-    T *__restrict__ dst = (T *)a.dst + (a.vec_off_out ? a.vec_off_out[vec] : vec * a.vec_stride_out);
+    const uint64_t voff_in = a.vec_off_in ? a.vec_off_in[vec] : vec * a.vec_stride_in;
+    const uint64_t voff_out = a.vec_off_out ? a.vec_off_out[vec] : vec * a.vec_stride_out;
+    const T *__restrict__ src = (const T *)a.src + voff_in;
+    T *__restrict__ dst = (T *)a.dst + voff_out;
+    const bool wide_in = sizeof(T) == 4 && (a.flags & kWideIn) != 0;
+    const uint64_t *__restrict__ srcw = (const uint64_t *)a.src + voff_in;
+    uint64_t *__restrict__ dstw = (uint64_t *)a.dst + voff_out;
     if constexpr (EPI == kEpiPrefix) {  // tiles whose p has no k' = p + P k < out_len
       if (((tile << logC) >> logS) >= (uint32_t)a.store_limit) return;
     }
@@ -290,7 +317,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     // raised issue priority while the tile's loads go out: a wave starting
     // a tile gets its memory requests in ahead of waves in their arithmetic
     __builtin_amdgcn_s_setprio(1);
-    load_tile<F, LOGR, LOGE, EPI>(x, src, a, tile, tid, c, t, staged, logC);
+    load_tile<F, LOGR, LOGE, EPI>(x, src, srcw, wide_in, a, tile, tid, c, t, staged, logC);
     __builtin_amdgcn_s_setprio(0);
 
     const uint32_t q = (tile << logC) + c;
@@ -445,22 +472,32 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     const bool pruned = EPI == kEpiGeneral && a.store_limit < (1ull << a.logN);
     const T sw = (T)a.scale_w;
     uint32_t nonzero = 0;
+    // WIDE: 64-bit output words (a uniform choice: the loop is compiled twice)
+    auto store_all = [&](auto wide_tag) {
+      constexpr bool WIDE = decltype(wide_tag)::value;
 #pragma unroll
-    for (int g = 0; g < E / EL; g++) {
+      for (int g = 0; g < E / EL; g++) {
 #pragma unroll
-      for (int kk = 0; kk < EL; kk++) {
-        const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
-        T v = x[g * EL + kk];
-        if (scale) v = F::mulw(v, sw);
-        const uint32_t oidx = (krow << logNR) + q;
-        if (EPI == kEpiGeneral && a.post_lo) {
-          const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
-          v = F::mulw(v, F::wmul(plo[oidx & ((1u << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
+        for (int kk = 0; kk < EL; kk++) {
+          const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
+          T v = x[g * EL + kk];
+          if (scale) v = F::mulw(v, sw);
+          const uint32_t oidx = (krow << logNR) + q;
+          if (EPI == kEpiGeneral && a.post_lo) {
+            const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
+            v = F::mulw(v, F::wmul(plo[oidx & ((1u << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
+          }
+          if (!pruned || (uint64_t)oidx < a.store_limit) {
+            if constexpr (WIDE) dstw[oidx] = (uint64_t)v;
+            else dst[oidx] = v;
+          } else {
+            nonzero += (v != 0);
+          }
         }
-        if (!pruned || (uint64_t)oidx < a.store_limit) dst[oidx] = v;
-        else nonzero += (v != 0);
       }
-    }
+    };
+    if (sizeof(T) == 4 && (a.flags & kWideOut) != 0) store_all(std::integral_constant<bool, sizeof(T) == 4>{});
+    else store_all(std::false_type{});
     if (EPI == kEpiGeneral && a.nonzero_count && nonzero) atomicAdd(a.nonzero_count, nonzero);
   }
 }

@@ -62,6 +62,8 @@ struct XformOpts {
   uint32_t *nonzero_count = nullptr;  // counts nonzero outputs at index >= store_limit
   const uint64_t *off_in = nullptr;   // device table: vector v's input at element off_in[v] (not in_stride)
   const uint64_t *off_out = nullptr;  // device table: vector v's output at element off_out[v] (not out_stride)
+  bool wide_in = false;   // 31/32-bit fields: input vectors are 64-bit words (value in the low half)
+  bool wide_out = false;  // 31/32-bit fields: output vectors are written as 64-bit words
   void *scratch[2] = {nullptr, nullptr};  // caller-owned intermediates (batch x N elements each), else the context's
   const struct TwTables *tables = nullptr;  // caller-built twiddle tables of (logN, root), else the context's cache
 };
@@ -86,6 +88,7 @@ struct omr_ctx {
   omr::DevBuf scratch[3];
   omr::DevBuf io[2];
   omr::DevBuf part, part2;
+  omr::DevBuf words[2];  // narrowed input / output of the word-width entry points
   // staging of the host-buffer forms (omr_intt, omr_ntt_batched,
   // omr_coset_lde, omr_coset_decode), under their own lock: they call the
   // device entry points, which take `mu`
@@ -173,7 +176,7 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
                          uint64_t root, const Schedule &sch, uint64_t batch, uint64_t in_stride,
                          uint64_t out_stride, const XformOpts &o, hipStream_t stream);
 omr_status forward_full(omr_ctx *ctx, int f, int variant, const void *d_in, void *d_out,
-                        uint32_t logN, uint64_t batch, uint64_t omega, hipStream_t stream);
+                        uint32_t logN, uint64_t batch, uint64_t omega, hipStream_t stream, bool wide = false);
 omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
                         uint64_t omega, uint64_t out_len, int mode, int method, uint32_t log_n1,
                         hipStream_t stream);
@@ -209,6 +212,9 @@ omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m,
 omr_status launch_sum_contig(int f, const void *part, void *out, uint64_t n, uint64_t m, uint32_t nvec,
                              uint64_t os, hipStream_t s);
 omr_status launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+// 64-bit words <-> 32-bit values: out32[i] = low half of in64[i] / out64[i] = in32[i]
+omr_status launch_narrow(const uint64_t *in, uint32_t *out, uint64_t n, hipStream_t s);
+omr_status launch_widen(const uint32_t *in, uint64_t *out, uint64_t n, hipStream_t s);
 omr_status launch_field_op(int f, int op, const void *a, const void *b, void *out, uint64_t n,
                            hipStream_t s);
 

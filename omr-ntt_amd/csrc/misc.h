@@ -222,6 +222,18 @@ __global__ void __launch_bounds__(256) k_scale(const typename F::T *__restrict__
   out[i] = apply ? F::mulw(x[i], w) : x[i];
 }
 
+__global__ void __launch_bounds__(256) k_narrow(const uint64_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)in[i];
+}
+
+__global__ void __launch_bounds__(256) k_widen(const uint32_t *__restrict__ in, uint64_t *__restrict__ out,
+                                               uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

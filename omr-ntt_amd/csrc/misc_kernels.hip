@@ -127,6 +127,18 @@ omr_status launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
   return last_launch();
 }
 
+omr_status launch_narrow(const uint64_t *in, uint32_t *out, uint64_t n, hipStream_t s) {
+  if (n == 0) return OMR_OK;
+  hipLaunchKernelGGL(k_narrow, dim3(blocks(n, 256)), dim3(256), 0, s, in, out, n);
+  return last_launch();
+}
+
+omr_status launch_widen(const uint32_t *in, uint64_t *out, uint64_t n, hipStream_t s) {
+  if (n == 0) return OMR_OK;
+  hipLaunchKernelGGL(k_widen, dim3(blocks(n, 256)), dim3(256), 0, s, in, out, n);
+  return last_launch();
+}
+
 omr_status launch_field_op(int f, int op, const void *a, const void *b, void *out, uint64_t n,
                            hipStream_t s) {
   if (n == 0) return OMR_OK;

@@ -94,6 +94,8 @@ def lib():
             "omr_variant_name": (ctypes.c_char_p, [i32]),
             "omr_variant_supports": (i32, [i32, u32]),
             "omr_ntt": (i32, [_VP, i32, i32, _VP, u64, u64, u64]),
+            "omr_ntt_words": (i32, [_VP, i32, i32, _VP, u64, u64, u64, u32]),
+            "omr_ntt_device_words": (i32, [_VP, i32, i32, _VP, _VP, u64, u64, u64, u64, u32, _VP]),
             "omr_ntt_device": (i32, [_VP, i32, i32, _VP, _VP, u64, u64, u64, u64, _VP]),
             "omr_intt_device": (i32, [_VP, i32, _VP, _VP, u64, u64, u64, _VP]),
             "omr_ntt_device_noalloc": (i32, [_VP, i32, i32, _VP, _VP, u64, u64, u64, i32, _VP]),
@@ -252,6 +254,32 @@ def ntt_device(x, out, field, variant=ARK_RADIX2, omega=None, s=0, batch=1, n=No
     _check(lib().omr_ntt_device(context(), field.id, variant, _dptr(x), _dptr(out), n, batch, omega, s,
                                 _stream_ptr()), "omr_ntt_device")
     return out
+
+
+def ntt_device_words(x, out, field, variant=ARK_RADIX2, omega=None, s=0, batch=1, n=None):
+    """ntt_device on int64 tensors holding one value of a 31/32-bit field per
+    64-bit word (the crate's one-limb storage)."""
+    n = n or x.numel() // batch
+    omega = root_of_unity(field, n) if omega is None else omega
+    for t in (x, out):
+        if not t.is_cuda or not t.is_contiguous() or t.element_size() != 8:
+            raise ValueError("word buffers must be contiguous 64-bit GPU tensors")
+    if x.numel() < n * batch:
+        raise ValueError("tensor too small")
+    _check(lib().omr_ntt_device_words(context(), field.id, variant, _dptr(x), _dptr(out), n, batch, omega, s, 8,
+                                      _stream_ptr()), "omr_ntt_device_words")
+    return out
+
+
+def ntt_words(buf, field, variant=ARK_RADIX2, omega=None, s=0):
+    """Host form: NttEncoder::ntt on a numpy uint64 array of 64-bit words, in place."""
+    if not isinstance(buf, np.ndarray) or buf.dtype != np.uint64 or not buf.flags["C_CONTIGUOUS"]:
+        raise TypeError("buf must be a contiguous numpy uint64 array")
+    n = buf.size
+    omega = root_of_unity(field, n) if omega is None else omega
+    _check(lib().omr_ntt_words(context(), field.id, variant, buf.ctypes.data_as(_VP), n, omega, s, 8),
+           "omr_ntt_words")
+    return buf
 
 
 def intt_device(x, out, field, omega=None, batch=1, n=None):
