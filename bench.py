@@ -269,13 +269,22 @@ def encoder(E, torch, np, with_model=True):
     res, cls, timed, sched = {}, None, 0, None
     if with_model:
         from omr_ntt import costmodel as CM
-        shapes = CM.shape_grid(fields=[E.BabyBear, E.Goldilocks], log_ns=range(12, 23, 2),
-                               batches=(1, 16, 64), partial=False, max_elems=1 << 27)
-        res, model = CM.experiment(shapes, reps=3, seed=0, steps=3000)
+        # trained on BabyBear + Goldilocks full transforms at batches 1, 16 and 256;
+        # regret on held-out shapes of that grid, on KoalaBear / TeddyBear
+        # (fields it never saw), on batch 64 (a batch it never saw) and on the
+        # encoder's own shape (2^22, 256 columns), each against exhaustive timing
+        enc_shape = (F, m, m, cols)
+        res, model = CM.experiment_extrapolation(reps=3, seed=0, steps=3000, extra_shapes=[enc_shape])
         # model-pruned search: the fastest of the model's 3 best-predicted
         # classes (3 timing runs instead of the exhaustive ~13)
         cls, timed = E.autotune_shape(F, m, m, cols, model=model, use_model=True, top_k=3)
         sched = E.schedule_class(F, m, m, cls)
+        ex = res["extra"]["per_shape"][0]
+        times = E.tune_classes(F, m, m, cols, reps=3)
+        best = times[times > 0].min()
+        res["encoder_shape"] = {"model_argmin_regret": ex["regret_model"], "default_regret": ex["regret_default"],
+                                "top3_pick_class": cls, "top3_pick_regret": float(times[cls] / best - 1.0),
+                                "best_class": int(np.argmin(np.where(times > 0, times, np.inf)))}
     coeffs = torch.randint(0, F.p, (n * cols,), dtype=torch.int32, device="cuda")
     evals = torch.empty(m * cols, dtype=torch.int32, device="cuda")
     back = torch.empty_like(coeffs)
@@ -288,13 +297,16 @@ def encoder(E, torch, np, with_model=True):
     enc_bytes = cols * (n + m) * 4
     log(f"bench: encoder 256x2^20 BabyBear blowup 4: encode {enc * 1e3:.2f} ms decode {dec * 1e3:.2f} ms "
         f"round trip ok={ok}; class {cls} {sched} ({timed} timed)"
-        + (f"; held-out regret {res['heldout_mean_regret_model']:.3f}" if with_model else ""))
+        + (f"; regret interp {res['interpolation']['mean_regret_model']:.3f}, unseen fields "
+           f"{res['unseen_fields']['mean_regret_model']:.3f}, unseen batch {res['unseen_batch_64']['mean_regret_model']:.3f}, "
+           f"encoder shape {res['encoder_shape']['top3_pick_regret']:.3f}" if with_model else ""))
     out = {"encode_ms": enc * 1e3, "decode_ms": dec * 1e3, "round_trip_ok": ok,
            "encode_alg_GBps": enc_bytes / enc / 1e9}
     if with_model:
         out.update({"schedule_class": cls, "schedule": sched,
                     "schedule_search": f"fastest of the cost model's top 3 ({timed} classes timed)",
-                    "cost_model": {k: v for k, v in res.items() if k != "per_shape"}})
+                    "cost_model": {k: ({kk: vv for kk, vv in v.items() if kk != "per_shape"} if isinstance(v, dict) else v)
+                                   for k, v in res.items() if k != "extra"}})
     return out
 
 

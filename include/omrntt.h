@@ -337,8 +337,9 @@ omr_status omr_tune_load(const char *path);
 /* ---- learned cost model -----------------------------------------------------
  * No counterpart in the reference.  A 4 -> 64 -> 64 -> omr_num_schedule_classes()
  * fp32 MLP (ReLU) whose outputs are predicted centred log-times of the
- * classes for a shape's features (field / 3, log n / 32, log out_len / 32,
- * log batch / 16); trained with Adam on autotuner timings by the engine's
+ * classes for a shape's features (the field as its element width,
+ * (bytes - 4) / 4; log n / 32; log out_len / 32; the batch as its total
+ * elements, log(n batch) / 40); trained with Adam on autotuner timings by the engine's
  * own HIP forward, backward and update kernels.  Parameters are
  * W1[64][4] b1[64] W2[64][64] b2[64] W3[K][64] b3[K], K = outputs. */
 typedef struct omr_cost_model omr_cost_model;

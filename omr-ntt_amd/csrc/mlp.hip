@@ -1,8 +1,12 @@
 // mlp.hip -- learned schedule cost model: a 4 -> 64 -> 64 -> 36 fp32 MLP
 // trained and run by the kernels below (VALU fp32; far too small for MFMA).
 //
-// Inputs (omr_cost_model_features): field id / 3, log2 N / 32,
-// log2 out_len / 32, log2 batch / 16.  Outputs: one score per schedule class
+// Inputs (omr_cost_model_features): the field as its element width
+// ((bytes - 4) / 4: 0 for the 31/32-bit fields, 1 for Goldilocks -- what
+// changes the kernels), log2 N / 32, log2 out_len / 32, and the batch as the
+// log2 of its total elements, log2(N batch) / 40 (what decides cache
+// residency), so that a field or batch size absent from the training shapes
+// maps onto features the model has seen.  Outputs: one score per schedule class
 // (tuning.hip), trained on autotuner timings (normalised log time); the
 // chosen schedule is the argmin over the classes valid for the shape.
 // Gradients are computed per sample into rows of a buffer and reduced in a
@@ -251,10 +255,10 @@ extern "C" omr_status omr_cost_model_features(int field, uint64_t n, uint64_t ou
   if (!x) return OMR_E_NULL;
   if (!field_ok(field)) return OMR_E_FIELD;
   if (n == 0 || out_len == 0 || batch == 0) return OMR_E_SIZE;
-  x[0] = (float)field / 3.f;
+  x[0] = (float)(field_bytes(field) - 4) / 4.f;
   x[1] = (float)std::log2((double)n) / 32.f;
   x[2] = (float)std::log2((double)out_len) / 32.f;
-  x[3] = (float)std::log2((double)batch) / 16.f;
+  x[3] = (float)(std::log2((double)n) + std::log2((double)batch)) / 40.f;
   return OMR_OK;
 } catch (const std::bad_alloc &) {
   return OMR_E_ALLOC;

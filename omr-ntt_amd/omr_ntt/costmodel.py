@@ -9,7 +9,7 @@ The method without the model is the engine's untuned default class.
 """
 import numpy as np
 
-from . import (CostModel, FIELDS, Goldilocks, BabyBear, default_class, features,
+from . import (CostModel, FIELDS, Goldilocks, BabyBear, KoalaBear, TeddyBear, default_class, features,
                num_schedule_classes, tune_classes)
 
 
@@ -95,3 +95,44 @@ def experiment(shapes=None, reps=3, seed=0, steps=3000):
         "timing_runs_model_per_shape": 0.0,
         "per_shape": held,
     }, model
+
+
+def _summary(held):
+    rm = np.array([r["regret_model"] for r in held])
+    rd = np.array([r["regret_default"] for r in held])
+    return {"shapes": len(held), "mean_regret_model": float(np.mean(rm)),
+            "mean_regret_default": float(np.nanmean(rd)),
+            "max_regret_model": float(np.max(rm)),
+            "frac_within_10pct_model": float(np.mean(rm <= 0.10)),
+            "frac_within_10pct_default": float(np.nanmean(rd <= 0.10))}
+
+
+def experiment_extrapolation(reps=3, seed=0, steps=3000, log_ns=range(12, 23, 2), max_elems=1 << 28,
+                             train_fields=(BabyBear, Goldilocks), held_fields=(KoalaBear, TeddyBear),
+                             train_batches=(1, 16, 256), held_batch=64, extra_shapes=()):
+    """Regret where the model has to extrapolate: trained on full transforms
+    of `train_fields` at `train_batches`, evaluated on (a) the same sizes in
+    fields it never saw (their field feature is new), (b) a batch size it
+    never saw, and (c) `extra_shapes` (e.g. the column encoder's shape),
+    each against exhaustive timing of every class."""
+    grid = shape_grid(list(train_fields), log_ns, train_batches, partial=False, max_elems=max_elems)
+    tr, te = split(grid)  # every 4th shape of the training grid held out: interpolation
+    tr_shapes = [grid[i] for i in tr]
+    groups = {
+        "interpolation": [grid[i] for i in te],
+        "unseen_fields": shape_grid(list(held_fields), log_ns, train_batches, partial=False, max_elems=max_elems),
+        f"unseen_batch_{held_batch}": shape_grid(list(train_fields), log_ns, (held_batch,), partial=False,
+                                                 max_elems=max_elems),
+    }
+    if extra_shapes:
+        groups["extra"] = list(extra_shapes)
+    X, times = collect(tr_shapes, reps)
+    model, loss = train(X, times, seed=seed, steps=steps)
+    res = {"train_shapes": len(tr_shapes), "train_fields": [F.name for F in train_fields],
+           "train_batches": list(train_batches), "final_loss": loss}
+    for name, shapes in groups.items():
+        _, t = collect(shapes, reps)
+        held = regrets(model, shapes, t)
+        res[name] = _summary(held)
+        res[name]["per_shape"] = held
+    return res, model

@@ -49,8 +49,13 @@ def test_schedule_classes_are_valid_decompositions():
 
 
 def test_features():
+    """The field enters as its element width, the batch as its total
+    elements: fields of one width share a feature value, and so do batches
+    of the same total size."""
     x = E.features(E.Goldilocks, 1 << 24, 1 << 8, 16)
-    assert np.allclose(x, [2 / 3, 24 / 32, 8 / 32, 4 / 16])
+    assert np.allclose(x, [1.0, 24 / 32, 8 / 32, 28 / 40])
+    for F in (E.BabyBear, E.KoalaBear, E.TeddyBear):
+        assert np.allclose(E.features(F, 1 << 20, 1 << 20, 64), [0.0, 20 / 32, 20 / 32, 26 / 40])
 
 
 def test_multipass_prefix_class_rule():
