@@ -172,13 +172,15 @@ __device__ __forceinline__ uint32_t lpad(uint32_t i) { return OMR_LDS_PAD ? i + 
 constexpr int lds_padded(int n) { return OMR_LDS_PAD ? n + n / 32 + 1 : n; }
 
 // In-tile stage ST >= 1 of the R-point Stockham: reads the previous stage's
-// outputs from LDS, applies w_R^(k' p s), runs the register DFTs.
+// outputs from LDS, applies w_R^(k' p s), runs the register DFTs.  The
+// previous stage's registers are written by thread (tw, cw) -- its own
+// (t, c), or for stage 1 of a row-fast pass the load mapping's.
 template <class F, int LOGR, int LOGE, int ST, int U = 0>
 __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
                                            const typename F::T (&wE)[(1 << LOGE) / 2 + 1],
                                            typename F::T *lds, const typename F::T *root,
                                            uint32_t t, uint32_t c, uint32_t logC,
-                                           uint32_t root_shift) {
+                                           uint32_t root_shift, uint32_t tw, uint32_t cw) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   if constexpr (ST < K::NSTAGE) {
@@ -188,7 +190,7 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
     constexpr int LOGS = LOGR - ST * LOGE - LOGEI;  // in-tile stride of this stage
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < K::E; m++) lds[lpad(((t + m * K::RE) << logC) + c)] = x[m];
+    for (int m = 0; m < K::E; m++) lds[lpad(((tw + m * K::RE) << logC) + cw)] = x[m];
     __syncthreads();
 #pragma unroll
     for (int g = 0; g < K::E / EI; g++) {
@@ -205,7 +207,7 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
       }
     }
     GroupDft<F, LOGE, LOGEI, K::E / EI, U>::run(x, wE);
-    tile_stage<F, LOGR, LOGE, ST + 1, U>(x, wE, lds, root, t, c, logC, root_shift);
+    tile_stage<F, LOGR, LOGE, ST + 1, U>(x, wE, lds, root, t, c, logC, root_shift, t, c);
   }
 }
 
@@ -290,6 +292,19 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   const uint32_t logS = a.logS;
   const uint32_t S = 1u << logS;  // s <= N/R <= 2^31
   const bool staged = S < C;
+  // A pass with s = 1 and at least two in-tile stages loads with the row
+  // fastest (thread (t0, c0) = (tid mod R/E, tid div R/E)): its columns are
+  // contiguous rows of R elements, so those loads are coalesced as they are,
+  // and stage 1's exchange through LDS hands the values to the column-fast
+  // mapping the stores need -- no staging transpose.  Other passes with
+  // s < C stage a contiguous block through LDS.  Goldilocks only: with
+  // 4-byte words the row-fast exchange writes conflict on the LDS banks and
+  // the pass got slower (BabyBear 2^24: 107 -> 120 us; Goldilocks 207 -> 204).
+  constexpr uint32_t LOGRE = LOGR - LOGE;
+  const bool rowfast = sizeof(T) == 8 && K::NSTAGE >= 2 && S == 1;
+  const uint32_t t0 = rowfast ? (tid & (RE - 1)) : t;
+  const uint32_t c0 = rowfast ? (tid >> LOGRE) : c;
+  const bool stage_lds = staged && !rowfast;
   // the pass's roots w_R^m, m < R: one contiguous block of the root pyramid
   const T *__restrict__ pyr = (const T *)a.root_pyr;
   const T *__restrict__ rootR = pyr + R;
@@ -317,14 +332,15 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     // raised issue priority while the tile's loads go out: a wave starting
     // a tile gets its memory requests in ahead of waves in their arithmetic
     __builtin_amdgcn_s_setprio(1);
-    load_tile<F, LOGR, LOGE, EPI>(x, src, srcw, wide_in, a, tile, tid, c, t, staged, logC);
+    load_tile<F, LOGR, LOGE, EPI>(x, src, srcw, wide_in, a, tile, tid, c0, t0, stage_lds, logC);
     __builtin_amdgcn_s_setprio(0);
 
-    const uint32_t q = (tile << logC) + c;
-    const uint32_t p = q >> logS, j = q & (S - 1);
+    // column, p and first input position of the load mapping
+    const uint32_t q0 = (tile << logC) + c0;
+    const uint32_t p = q0 >> logS, j = q0 & (S - 1);
     const uint32_t pos0 = ((p << LOGR) << logS) + j;
 
-    if (staged) {
+    if (stage_lds) {
       // element i = (pp R + k) S + jj of the block is column pp S + jj, row k
       __syncthreads();
 #pragma unroll
@@ -345,7 +361,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
       const uint32_t lomask = (1u << a.lo_bits) - 1;
 #pragma unroll
       for (int m = 0; m < E; m++) {
-        const uint32_t pos = pos0 + ((t + m * RE) << logS);
+        const uint32_t pos = pos0 + ((t0 + m * RE) << logS);
         x[m] = F::mulw(x[m], F::wmul(plo[pos & lomask], phi[pos >> a.lo_bits]));
       }
     }
@@ -354,7 +370,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     if (a.flags & kTwiddle) {  // w_N^(k p s) = w_(N/s)^(k p) on input row k
       const uint32_t logNs = a.logN - logS;
       const bool one = logNs <= a.log_mid;
-      // Rows k = t + m RE: w^(k p) = A B^m with A = w^(t p), B = w^(RE p), in
+      // Rows k = t0 + m RE: w^(k p) = A B^m with A = w^(t0 p), B = w^(RE p), in
       // two chains (even / odd m) stepped by B^2: two table lookups per
       // thread instead of one or two per element.  Used where it saves loads
       // at no extra products (the two-level case), and for 32-bit fields
@@ -370,7 +386,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
           const uint32_t en = e << logS;  // as a power of w_N
           return F::wmul(tlo[en & lomask], thi[en >> a.lo_bits]);
         };
-        T A = tw(t * p);
+        T A = tw(t0 * p);
         if (EPI == kEpiScale) {  // the inverse's scale rides in A: once per element
           A = F::wmul(A, (T)a.scale_w);
           scaled_in = true;
@@ -386,30 +402,22 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
             if (m + 2 < E) ch[r] = F::wmul(ch[r], B2);
           }
         }
-      } else if (one) {  // one lookup: w_(N/s)^e from the root pyramid or the mid table
+      } else {  // Goldilocks, one lookup: w_(N/s)^e from the root pyramid or the mid table
         const bool small = logNs <= a.log_rmax;
         const T *tab = small ? pyr + (1u << logNs) : (const T *)a.tw_mid;
         const uint32_t sh = small ? 0u : a.log_mid - logNs;
 #pragma unroll
         for (int m = 0; m < E; m++) {
-          const uint32_t e = (t + m * RE) * p;
+          const uint32_t e = (t0 + m * RE) * p;
           x[m] = F::mulw(x[m], tab[e << sh]);  // e = 0 multiplies by 1: no divergent branch
-        }
-      } else {  // two-level table (e < N <= 2^32)
-        const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
-        const uint32_t ps = p << logS;
-        const uint32_t lomask = (1u << a.lo_bits) - 1;
-#pragma unroll
-        for (int m = 0; m < E; m++) {
-          const uint32_t e = (t + m * RE) * ps;
-          x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]));
         }
       }
     }
 
     // in-tile stage 0: stride R/E, one group per thread, no in-tile twiddle
     dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0, U>(x, wE);
-    tile_stage<F, LOGR, LOGE, 1, U>(x, wE, lds, rootR, t, c, logC, 0);
+    tile_stage<F, LOGR, LOGE, 1, U>(x, wE, lds, rootR, t, c, logC, 0, t0, c0);
+    const uint32_t q = (tile << logC) + c;  // the store mapping's column
 
     // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
     constexpr int EL = 1 << K::LOGLAST;
