@@ -18,9 +18,11 @@ inline bool has_pass_kernel(int logR, int logE, int esize) {
   return logR - logE <= 10;  // one thread per E elements of a column: <= 1024 threads
 }
 
-// Columns per tile the kernel for (esize, logR) was compiled for.
-inline int pass_cmax(int logR, int esize) {
-  int c = (kLdsBudget / esize) >> logR;
+// Columns per tile the kernel for (esize, logR, logE) was compiled for
+// (engine.h: PassKernel::CMAX, wide tiles included).
+inline int pass_cmax(int logR, int esize, int logE = 0) {
+  const bool wide = esize == 4 && (logR == 10 || logR == 11) && logE == 4;
+  int c = ((wide ? kLdsBudgetWide : kLdsBudget) / esize) >> logR;
   if (c < 1) c = 1;
   if (c > 32) c = 32;
   return c;
@@ -30,7 +32,7 @@ inline int pass_cmax(int logR, int esize) {
 // number of columns, and at most 1024 threads per workgroup.
 inline int pass_log_cols(int logN, int logR, int logE, int esize) {
   int lc = 0;
-  while ((2 << lc) <= pass_cmax(logR, esize)) lc++;
+  while ((2 << lc) <= pass_cmax(logR, esize, logE)) lc++;
   if (lc > logN - logR) lc = logN - logR;
   if (lc > 10 - (logR - logE)) lc = 10 - (logR - logE);
   return lc < 0 ? 0 : lc;

@@ -149,6 +149,25 @@ struct GroupDft<F, LOGE, LOGX, 0, U> {
 #endif
 constexpr int kLdsBudget = OMR_LDS_BUDGET;  // bytes of tile data per workgroup
 
+// Wide tiles: 4-byte fields at radix 2^10 / 2^11 get 64 KiB of tile data
+// (16 / 8 columns, 64 / 32-byte runs) instead of 8 / 4 columns, so that a
+// 2^22 transform runs in two HBM passes of radix 2^11 with runs the L2
+// merges (tiles in XCD order, see xcd_tile); 1024 threads at register radix
+// 16.
+constexpr int kLdsBudgetWide = 65536;
+template <class T, int LOGR, int LOGE>
+constexpr bool wide_tile() {
+  return sizeof(T) == 4 && (LOGR == 10 || LOGR == 11) && LOGE == 4;
+}
+
+// Tile of workgroup b in a grid of G when consecutive tiles share cache
+// lines: the hardware deals workgroups to the 8 XCDs round robin (b mod 8),
+// so tile (b mod 8) G/8 + b/8 puts consecutive tiles on one XCD at about
+// the same time; the grid must be a multiple of 8, else tiles stay in order.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
+  return (G & 7u) ? b : (b & 7u) * (G >> 3) + (b >> 3);
+}
+
 template <class F, int LOGR, int LOGE>
 struct PassKernel {
   using T = typename F::T;
@@ -157,7 +176,8 @@ struct PassKernel {
   static constexpr int RE = R / E;
   static constexpr int NSTAGE = (LOGR + LOGE - 1) / LOGE;
   static constexpr int LOGLAST = LOGR - (NSTAGE - 1) * LOGE;  // radix of the last in-tile stage
-  static constexpr int CMAX0 = (kLdsBudget / (int)sizeof(T)) >> LOGR;
+  static constexpr bool WIDE = wide_tile<T, LOGR, LOGE>();
+  static constexpr int CMAX0 = ((WIDE ? kLdsBudgetWide : kLdsBudget) / (int)sizeof(T)) >> LOGR;
   static constexpr int CMAX = CMAX0 < 1 ? 1 : (CMAX0 > 32 ? 32 : CMAX0);
   static constexpr int MAXT = CMAX * RE > 1024 ? 1024 : CMAX * RE;  // hardware limit
   static constexpr int LDS_N = CMAX * R;
@@ -315,7 +335,8 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   load_consts<F, LOGE, U>(wE, rootR, RE, 0);
 
   {
-    const uint64_t gt = blockIdx.x;  // grid = nvec << logTiles tiles
+    // grid = nvec << logTiles tiles; wide tiles (short runs) in XCD order
+    const uint64_t gt = K::WIDE ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint64_t vec = gt >> logTiles;
     const uint32_t tile = (uint32_t)(gt & ((1ull << logTiles) - 1));
     const uint64_t voff_in = a.vec_off_in ? a.vec_off_in[vec] : vec * a.vec_stride_in;
