@@ -502,6 +502,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--logn", type=int, default=24)
+    ap.add_argument("--schedule", default=None,
+                    help="comma-separated log2 pass radices for the headline transform (e.g. 12,12 for the "
+                         "four-step 2^12 x 2^12), to profile a schedule; default: the engine's tuned one")
     ap.add_argument("--prewarm-s", type=float, default=0.5,
                     help="untimed transforms for this many seconds before --warmup (clock ramp-up)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -558,7 +561,7 @@ def main():
 
     F = E.Goldilocks
     n = 1 << args.logn
-    sched = E.default_schedule(F, n)
+    sched = [int(v) for v in args.schedule.split(",")] if args.schedule else E.default_schedule(F, n)
     npass = len(sched)
     log(f"bench: Goldilocks 2^{args.logn} forward NTT, schedule {sched}, steps={args.steps} warmup={args.warmup}")
 
@@ -570,7 +573,10 @@ def main():
     ys = [E.empty_device(n, F) for _ in range(nbuf)]
 
     def fwd(i):
-        E.ntt_device(xs[i % nbuf], ys[i % nbuf], F)
+        if args.schedule:
+            E.ntt_schedule_device(xs[i % nbuf], ys[i % nbuf], F, sched)
+        else:
+            E.ntt_device(xs[i % nbuf], ys[i % nbuf], F)
 
     prewarm_calls = prewarm(fwd, torch, args.prewarm_s)
     sec = time_device(fwd, args.steps, args.warmup, torch)
@@ -585,7 +591,10 @@ def main():
         round_trip &= bool(torch.equal(back, xs[i]))
 
     def inv(i):
-        E.intt_device(ys[i % nbuf], xs[i % nbuf], F)
+        if args.schedule:
+            E.ntt_schedule_device(ys[i % nbuf], xs[i % nbuf], F, sched, inverse=True)
+        else:
+            E.intt_device(ys[i % nbuf], xs[i % nbuf], F)
 
     # intt(ntt(x)) == x, so the inverse steps rewrite the inputs unchanged
     prewarm(inv, torch, min(args.prewarm_s, 0.2))

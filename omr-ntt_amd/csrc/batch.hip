@@ -285,7 +285,7 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
       p->pinned.emplace_back(field, logn, w);
       bt.lo = tab->lo;
       bt.hi = tab->hi;
-      bt.root = tab->root;
+      bt.pyr = tab->pyr;
       bt.lo_bits = tab->lo_bits;
       bt.log_rmax = tab->log_rmax;
       bt.mid = tab->mid;

@@ -30,7 +30,7 @@ struct BatchVec {  // one vector in one pass: buffer ids (0 in, 1 out, 2 A, 3 B)
 };
 
 struct BatchTables {  // tables of one root of order 2^logN, multiplier form
-  const void *lo, *hi, *root, *mid;
+  const void *lo, *hi, *pyr, *mid;  // pyr: the root pyramid (w_(2^L)^m at [2^L + m], L <= log_rmax)
   uint64_t scale_w;
   uint32_t lo_bits, log_rmax, log_mid;
   uint32_t unit;  // register-DFT constants: 0 = table constants, 39 / 153 = 2^k shifts (Goldilocks)
@@ -44,7 +44,7 @@ template <class F, int LEI, int U>
 __device__ __forceinline__ void gen_stage(typename F::T (&x)[16], const typename F::T (&wE)[9],
                                           const typename F::T *lds, const typename F::T *root,
                                           uint32_t t, uint32_t c, uint32_t logC, uint32_t RE,
-                                          uint32_t logs, uint32_t root_shift) {
+                                          uint32_t logs) {
   using T = typename F::T;
   constexpr int EI = 1 << LEI;
 #pragma unroll
@@ -57,7 +57,7 @@ __device__ __forceinline__ void gen_stage(typename F::T (&x)[16], const typename
       const uint32_t row = ps * EI + ((uint32_t)kk << logs) + jj;
       T v = lds[lpad((row << logC) + c)];
       const uint32_t e = (uint32_t)kk * ps;
-      if (kk != 0) v = F::mulw(v, root[e << root_shift]);  // e = 0: times root[0] = 1
+      if (kk != 0) v = F::mulw(v, root[e]);  // e = 0: times root[0] = 1
       x[g * EI + kk] = v;
     }
   }
@@ -70,9 +70,10 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
                                            const BatchVec *__restrict__ vecs, const BatchBases &bases,
                                            typename F::T *lds, uint32_t tid) {
   using T = typename F::T;
-  const T *__restrict__ root = (const T *)tb.root;
+  const T *__restrict__ pyr = (const T *)tb.pyr;
   const uint32_t logN = d.logN, logR = d.logR, logS = d.logS;
   const uint32_t R = 1u << logR, RE = R >> 4;
+  const T *__restrict__ root = pyr + R;  // w_R^m, m < R: one contiguous block
   const uint32_t logC = 12 - logR;  // C = 4096 / R columns per task
   const uint32_t c = tid & ((1u << logC) - 1), t = tid >> logC;
   const uint32_t logCols = logN - logR;  // columns per vector
@@ -86,10 +87,8 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
   const BatchVec bv = vecs[d.vec0 + (active ? u : 0)];
   const T *__restrict__ src = (const T *)bases.buf[bv.src_buf] + bv.src_off;
   T *__restrict__ dst = (T *)bases.buf[bv.dst_buf] + bv.dst_off;
-  const uint32_t root_shift = tb.log_rmax - logR;
-
   T wE[9];
-  load_consts<F, 4, U>(wE, root, RE, root_shift);
+  load_consts<F, 4, U>(wE, root, RE, 0);
 
   T x[16];
   if (S >= (1u << logCv)) {
@@ -117,23 +116,34 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
     for (int m = 0; m < 16; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
   }
 
-  if (d.flags & kTwiddle) {  // w_N^(k p s) = w_(N/s)^(k p) on input row k
+  if (d.flags & kTwiddle) {  // w_(N/s)^(k p) on input row k = t + m RE
     const uint32_t logNs = logN - logS;
-    if (logNs <= tb.log_mid) {  // one lookup from the root or the mid table
-      const bool small = logNs <= tb.log_rmax;
-      const T *tab = small ? root : (const T *)tb.mid;
-      const uint32_t sh = (small ? tb.log_rmax : tb.log_mid) - logNs;
+    const bool one = logNs <= tb.log_mid;
+    const bool small = logNs <= tb.log_rmax;
+    const T *tab = small ? pyr + (1u << logNs) : (const T *)tb.mid;
+    const uint32_t sh = small ? 0u : tb.log_mid - logNs;
+    if (sizeof(T) == 4 || !one) {  // power chains A B^m, as in the pass engine
+      const T *tlo = (const T *)tb.lo, *thi = (const T *)tb.hi;
+      const uint32_t lomask = (1u << tb.lo_bits) - 1;
+      auto tw = [&](uint32_t e) -> T {  // w_(N/s)^e, e < N/s
+        if (one) return tab[e << sh];
+        const uint32_t en = e << logS;
+        return F::wmul(tlo[en & lomask], thi[en >> tb.lo_bits]);
+      };
+      const T A = tw(t * p), B = tw(RE * p);
+      const T B2 = F::wmul(B, B);
+      T ch[2] = {A, F::wmul(A, B)};
+#pragma unroll
+      for (int m = 0; m < 16; m += 2) {
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+          x[m + r] = F::mulw(x[m + r], ch[r]);
+          if (m + 2 < 16) ch[r] = F::wmul(ch[r], B2);
+        }
+      }
+    } else {
 #pragma unroll
       for (int m = 0; m < 16; m++) x[m] = F::mulw(x[m], tab[((t + m * RE) * p) << sh]);
-    } else {
-      const T *tlo = (const T *)tb.lo, *thi = (const T *)tb.hi;
-      const uint32_t ps = p << logS;
-      const uint32_t lomask = (1u << tb.lo_bits) - 1;
-#pragma unroll
-      for (int m = 0; m < 16; m++) {
-        const uint32_t e = (t + m * RE) * ps;
-        x[m] = F::mulw(x[m], F::wmul(tlo[e & lomask], thi[e >> tb.lo_bits]));
-      }
     }
   }
 
@@ -148,10 +158,10 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
     for (int m = 0; m < 16; m++) lds[lpad(((t + m * RE) << logC) + c)] = x[m];
     __syncthreads();
     switch (logEi) {
-      case 4: gen_stage<F, 4, U>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
-      case 3: gen_stage<F, 3, U>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
-      case 2: gen_stage<F, 2, U>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
-      default: gen_stage<F, 1, U>(x, wE, lds, root, t, c, logC, RE, logs, root_shift); break;
+      case 4: gen_stage<F, 4, U>(x, wE, lds, root, t, c, logC, RE, logs); break;
+      case 3: gen_stage<F, 3, U>(x, wE, lds, root, t, c, logC, RE, logs); break;
+      case 2: gen_stage<F, 2, U>(x, wE, lds, root, t, c, logC, RE, logs); break;
+      default: gen_stage<F, 1, U>(x, wE, lds, root, t, c, logC, RE, logs); break;
     }
   }
 
@@ -181,9 +191,14 @@ k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchV
   __shared__ T lds[lds_padded(kTileElems)];
   const uint32_t tid = threadIdx.x;
 
+  // the next task's descriptor is loaded while this one runs: its tables,
+  // vector entry and data loads then start without a chain of dependent
+  // descriptor latencies
+  BatchTask dn = tasks[blockIdx.x < ntasks ? blockIdx.x : 0];
   for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
-    const BatchTask d = tasks[ti];
+    const BatchTask d = dn;
     const BatchTables tb = tsets[d.tset];
+    if (ti + gridDim.x < ntasks) dn = tasks[ti + gridDim.x];
     if constexpr (F::kPow2Roots) {  // per-task register-DFT constants: forward / inverse roots
       if (tb.unit == 39) batch_task<F, 39>(d, tb, vecs, bases, lds, tid);
       else if (tb.unit == 153) batch_task<F, 153>(d, tb, vecs, bases, lds, tid);
