@@ -78,23 +78,32 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
   const uint32_t c = tid & ((1u << logC) - 1), t = tid >> logC;
   const uint32_t logCols = logN - logR;  // columns per vector
   const uint32_t logCv = logCols < logC ? logCols : logC;
-  const uint32_t u = c >> logCv;  // vector within the task
-  const bool active = u < d.nvec;
-  // offsets inside one vector are 32-bit (N <= 2^32)
-  const uint32_t q = (d.tile << logC) + (c & ((1u << logCv) - 1));
   const uint32_t S = 1u << logS;
-  const uint32_t p = q >> logS, j = q & (S - 1);
-  const BatchVec bv = vecs[d.vec0 + (active ? u : 0)];
+  // A pass with s = 1 and at least two in-tile stages (Goldilocks) loads
+  // with the row fastest, thread (t0, c0) = (tid mod R/16, tid div R/16): a
+  // column is a contiguous run of R elements, so the loads are coalesced as
+  // they are and stage 1's LDS exchange hands the values to the column-fast
+  // mapping of the stores; no staging transpose (whose 8-byte column writes
+  // conflict on the LDS banks).  Other passes with s < C stage a contiguous
+  // block through LDS.
+  const bool rowfast = sizeof(T) == 8 && logR >= 5 && logS == 0;
+  const uint32_t t0 = rowfast ? (tid & (RE - 1)) : t;
+  const uint32_t c0 = rowfast ? (tid >> (logR - 4)) : c;
+  // offsets inside one vector are 32-bit (N <= 2^32)
+  const uint32_t u0 = c0 >> logCv;  // vector of the load mapping's column
+  const bool active0 = u0 < d.nvec;
+  const uint32_t q0 = (d.tile << logC) + (c0 & ((1u << logCv) - 1));
+  const uint32_t p = q0 >> logS, j = q0 & (S - 1);
+  const BatchVec bv = vecs[d.vec0 + (active0 ? u0 : 0)];
   const T *__restrict__ src = (const T *)bases.buf[bv.src_buf] + bv.src_off;
-  T *__restrict__ dst = (T *)bases.buf[bv.dst_buf] + bv.dst_off;
   T wE[9];
   load_consts<F, 4, U>(wE, root, RE, 0);
 
   T x[16];
-  if (S >= (1u << logCv)) {
+  if (rowfast || S >= (1u << logCv)) {
     const T *base = src + (((p << logR) << logS) + j);
 #pragma unroll
-    for (int m = 0; m < 16; m++) x[m] = active ? base[(t + m * RE) << logS] : (T)0;
+    for (int m = 0; m < 16; m++) x[m] = active0 ? base[(t0 + m * RE) << logS] : (T)0;
   } else {
     // each vector of the task contributes the contiguous block
     // [q0 R, (q0 + Cv) R) of its data, q0 = tile * C
@@ -116,7 +125,7 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
     for (int m = 0; m < 16; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
   }
 
-  if (d.flags & kTwiddle) {  // w_(N/s)^(k p) on input row k = t + m RE
+  if (d.flags & kTwiddle) {  // w_(N/s)^(k p) on input row k = t0 + m RE
     const uint32_t logNs = logN - logS;
     const bool one = logNs <= tb.log_mid;
     const bool small = logNs <= tb.log_rmax;
@@ -130,7 +139,7 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
         const uint32_t en = e << logS;
         return F::wmul(tlo[en & lomask], thi[en >> tb.lo_bits]);
       };
-      const T A = tw(t * p), B = tw(RE * p);
+      const T A = tw(t0 * p), B = tw(RE * p);
       const T B2 = F::wmul(B, B);
       T ch[2] = {A, F::wmul(A, B)};
 #pragma unroll
@@ -143,7 +152,7 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
       }
     } else {
 #pragma unroll
-      for (int m = 0; m < 16; m++) x[m] = F::mulw(x[m], tab[((t + m * RE) * p) << sh]);
+      for (int m = 0; m < 16; m++) x[m] = F::mulw(x[m], tab[((t0 + m * RE) * p) << sh]);
     }
   }
 
@@ -153,9 +162,10 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
   for (uint32_t st = 1; st < nst; st++) {
     const uint32_t logEi = (st == nst - 1) ? loglast : 4;
     const uint32_t logs = logR - 4 * st - logEi;
+    const uint32_t tw = st == 1 ? t0 : t, cw = st == 1 ? c0 : c;  // stage 1: registers of the load mapping
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 16; m++) lds[lpad(((t + m * RE) << logC) + c)] = x[m];
+    for (int m = 0; m < 16; m++) lds[lpad(((tw + m * RE) << logC) + cw)] = x[m];
     __syncthreads();
     switch (logEi) {
       case 4: gen_stage<F, 4, U>(x, wE, lds, root, t, c, logC, RE, logs); break;
@@ -170,6 +180,12 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
 #pragma unroll
     for (int m = 0; m < 16; m++) x[m] = F::mulw(x[m], sw);
   }
+  // stores: column-fast mapping (t, c)
+  const uint32_t u = c >> logCv;
+  const bool active = u < d.nvec;
+  const uint32_t q = (d.tile << logC) + (c & ((1u << logCv) - 1));
+  const BatchVec bs = rowfast ? vecs[d.vec0 + (active ? u : 0)] : bv;
+  T *__restrict__ dst = (T *)bases.buf[bs.dst_buf] + bs.dst_off;
   const uint32_t logNR = logN - logR;
   const uint32_t ELm = (1u << loglast) - 1;
 #pragma unroll
