@@ -125,6 +125,7 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
     for (int m = 0; m < 16; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
   }
 
+  bool scaled = false;  // the inverse's 1/N folded into the twiddle chain
   if (d.flags & kTwiddle) {  // w_(N/s)^(k p) on input row k = t0 + m RE
     const uint32_t logNs = logN - logS;
     const bool one = logNs <= tb.log_mid;
@@ -139,7 +140,12 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
         const uint32_t en = e << logS;
         return F::wmul(tlo[en & lomask], thi[en >> tb.lo_bits]);
       };
-      const T A = tw(t0 * p), B = tw(RE * p);
+      T A = tw(t0 * p);
+      const T B = tw(RE * p);
+      if (d.flags & kScale) {
+        A = F::wmul(A, (T)tb.scale_w);
+        scaled = true;
+      }
       const T B2 = F::wmul(B, B);
       T ch[2] = {A, F::wmul(A, B)};
 #pragma unroll
@@ -175,7 +181,7 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
     }
   }
 
-  if (d.flags & kScale) {  // a block of its own: the store loop stays branch-free
+  if ((d.flags & kScale) && !scaled) {  // a block of its own: the store loop stays branch-free
     const T sw = (T)tb.scale_w;
 #pragma unroll
     for (int m = 0; m < 16; m++) x[m] = F::mulw(x[m], sw);
