@@ -57,6 +57,17 @@ inline int pass_epilogue(const PassArgs &a) {
   return (a.flags & kScale) ? kEpiScale : kEpiPlain;
 }
 
+// Whether a kEpiGeneral pass is the first pass of a blowup-4 coset
+// extension that kEpiLde runs: zero tail from N/4 on, an input multiplier,
+// nothing else; register radix 16 with at least two in-tile stages; tiles
+// whose rows are strided (s >= C).  The pass kernel's own column count
+// check (compile-time C) comes on top.
+inline bool lde4_ok(const PassArgs &a, int logR, int logE) {
+  return logE == 4 && logR >= 5 && logR <= 12 && (a.flags & kZeroTail) && a.logN == a.logValid + 2 &&
+         a.logS + (uint32_t)logR == a.logN && a.logS >= a.logC && a.pre_lo && !a.post_lo && !a.nonzero_count &&
+         a.store_limit >= (1ull << a.logN) && !(a.flags & (kScale | kTwiddle | kPrefixSum));
+}
+
 // Goldilocks radix-4096 passes as two 64-point register DFTs (fourstep.h):
 // whether a pass qualifies (forward / inverse ark roots, 2^15 <= N <= 2^29,
 // 8 columns of one p per tile, plain or scaled epilogue), and its launch

@@ -63,7 +63,13 @@ enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u, k
 //                  j = q mod s times w_N^(j k'), summed over the tile's
 //                  columns: one partial sum per (k', tile) instead of the
 //                  pass output
-enum : int { kEpiPlain = 0, kEpiScale = 1, kEpiGeneral = 2, kEpiPrefix = 3 };
+//   kEpiLde     -- the first pass of a blowup-4 coset extension (zero tail
+//                  from N/4 on, input multiplier h^pos, plain store) at
+//                  register radix 16: a thread's rows t + m R/16 are zero
+//                  for m >= 4, so it loads and shifts 4 rows and its first
+//                  register DFT skips the two butterfly levels that only
+//                  see zeros
+enum : int { kEpiPlain = 0, kEpiScale = 1, kEpiGeneral = 2, kEpiPrefix = 3, kEpiLde = 4 };
 constexpr int kPrefixMaxLogR = 10;  // radices compiled with the prefix epilogue
 
 template <int LOGX>
@@ -101,15 +107,16 @@ __device__ __forceinline__ void load_consts(typename F::T (&wE)[(1 << LOGE) / 2 
 // In-register DFT of size X = 2^LOGX on x[OFF .. OFF+X), natural order in
 // and out.  wE[k] = w_E^k for the register radix E = 2^LOGE; the sub-DFT
 // uses w_X = w_E^(E/X).  Radix-2 decimation in frequency followed by a
-// compile-time register permutation (free).
-template <class F, int LOGE, int LOGX, int OFF, int U = 0>
+// compile-time register permutation (free).  LEN0 < X: the butterfly
+// levels of length > LEN0 are already done (dft_reg_nz4).
+template <class F, int LOGE, int LOGX, int OFF, int U = 0, int LEN0 = (1 << LOGX)>
 __device__ __forceinline__ void dft_reg(typename F::T (&x)[1 << LOGE],
                                         const typename F::T (&wE)[(1 << LOGE) / 2 + 1]) {
   using T = typename F::T;
   constexpr int X = 1 << LOGX;
   constexpr int E = 1 << LOGE;
 #pragma unroll
-  for (int len = X; len >= 2; len >>= 1) {
+  for (int len = LEN0; len >= 2; len >>= 1) {
     const int half = len >> 1;
 #pragma unroll
     for (int b = 0; b < X; b += len) {
@@ -127,6 +134,30 @@ __device__ __forceinline__ void dft_reg(typename F::T (&x)[1 << LOGE],
   for (int i = 0; i < X; i++) y[brev_c<LOGX>(i)] = x[OFF + i];
 #pragma unroll
   for (int i = 0; i < X; i++) x[OFF + i] = y[i];
+}
+
+// dft_reg of the full register radix E = 2^LOGE >= 16 when x[m] = 0 for
+// m >= E/4 (canonical x[m] for m < E/4): the first level's differences are
+// u w^j (v = 0) and its sums u, and the second level sees a zero v again;
+// only the twiddle products of those two levels remain.  Same output as
+// dft_reg on the zero-padded input.
+template <class F, int LOGE, int U = 0>
+__device__ __forceinline__ void dft_reg_nz4(typename F::T (&x)[1 << LOGE],
+                                            const typename F::T (&wE)[(1 << LOGE) / 2 + 1]) {
+  using T = typename F::T;
+  constexpr int E = 1 << LOGE, Q = E / 4;
+  static_assert(LOGE >= 2, "register radix of at least 4");
+  const T zero = 0;
+#pragma unroll
+  for (int j = 0; j < Q; j++)  // level len = E: (u, 0) -> (u, u w_E^j)
+    x[j + 2 * Q] = (j == 0) ? x[j] : diff_mul<F, U, LOGE>(x[j], zero, wE[j], j);
+#pragma unroll
+  for (int b = 0; b < E; b += 2 * Q) {  // level len = E/2: (u, 0) -> (u, u w_(E/2)^j)
+#pragma unroll
+    for (int j = 0; j < Q; j++)
+      x[b + j + Q] = (j == 0) ? x[b + j] : diff_mul<F, U, LOGE>(x[b + j], zero, wE[2 * j], 2 * j);
+  }
+  dft_reg<F, LOGE, LOGE, 0, U, E / 4>(x, wE);
 }
 
 // E / X independent sub-DFTs of size X = 2^LOGX on consecutive register groups.
@@ -262,6 +293,19 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
     pos0 = (tile << (logC + LOGR)) + tid;
     step = (1u << logC) * K::RE;
   }
+  if constexpr (EPI == kEpiLde) {  // rows m >= E/4 lie in the zero tail
+#pragma unroll
+    for (int m = 0; m < K::E; m++) {
+      if (m < K::E / 4) {
+        const uint32_t pos = pos0 + (uint32_t)m * step;
+        if constexpr (sizeof(T) == 4) xn[m] = wide ? (T)ld_stream(srcw + pos) : ld_stream(src + pos);
+        else xn[m] = ld_stream(src + pos);
+      } else {
+        xn[m] = 0;
+      }
+    }
+    return;
+  }
   if constexpr (sizeof(T) == 4) {
     if (wide) {  // 64-bit words: the value is the low half
       if (EPI == kEpiGeneral && (a.flags & kZeroTail)) {
@@ -377,11 +421,11 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
       for (int m = 0; m < E; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
     }
 
-    if (EPI == kEpiGeneral && a.pre_lo) {  // h^pos on input position pos (coset shift)
+    if ((EPI == kEpiGeneral || EPI == kEpiLde) && a.pre_lo) {  // h^pos on input position pos (coset shift)
       const T *plo = (const T *)a.pre_lo, *phi = (const T *)a.pre_hi;
       const uint32_t lomask = (1u << a.lo_bits) - 1;
 #pragma unroll
-      for (int m = 0; m < E; m++) {
+      for (int m = 0; m < (EPI == kEpiLde ? E / 4 : E); m++) {
         const uint32_t pos = pos0 + ((t0 + m * RE) << logS);
         x[m] = F::mulw(x[m], F::wmul(plo[pos & lomask], phi[pos >> a.lo_bits]));
       }
@@ -436,7 +480,8 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     }
 
     // in-tile stage 0: stride R/E, one group per thread, no in-tile twiddle
-    dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0, U>(x, wE);
+    if constexpr (EPI == kEpiLde && K::NSTAGE >= 2) dft_reg_nz4<F, LOGE, U>(x, wE);
+    else dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0, U>(x, wE);
     tile_stage<F, LOGR, LOGE, 1, U>(x, wE, lds, rootR, t, c, logC, 0, t0, c0);
     const uint32_t q = (tile << logC) + c;  // the store mapping's column
 

@@ -154,7 +154,8 @@ hipError_t launch_pass(int field, int logR, int logE, int unit, const PassArgs &
                        dim3 block, hipStream_t stream) {
   // the pass reads its roots from the pyramid's level logR
   if (!a.root_pyr || logR < 0 || (uint32_t)logR > a.log_rmax) return hipErrorInvalidValue;
-  const int epi = pass_epilogue(a);
+  int epi = pass_epilogue(a);
+  if (epi == kEpiGeneral && lde4_ok(a, logR, logE)) epi = kEpiLde;
   if (fs4096_ok(field, logR, unit, a.logN, a.logS, epi))
     return launch_pass_fs4096(unit, epi, a, (((uint64_t)1 << a.logN) >> 15) * a.nvec, stream);
   switch (field) {
