@@ -269,7 +269,7 @@ def encoder(E, torch, np, with_model=True):
     res, cls, timed, sched = {}, None, 0, None
     if with_model:
         from omr_ntt import costmodel as CM
-        # trained on BabyBear + Goldilocks full transforms at batches 1, 16 and 256;
+        # trained on BabyBear + Goldilocks full transforms, log n 11..22, at batches 1, 16 and 256;
         # regret on held-out shapes of that grid, on KoalaBear / TeddyBear
         # (fields it never saw), on batch 64 (a batch it never saw) and on the
         # encoder's own shape (2^22, 256 columns), each against exhaustive timing

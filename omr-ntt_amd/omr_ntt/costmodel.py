@@ -107,14 +107,17 @@ def _summary(held):
             "frac_within_10pct_default": float(np.nanmean(rd <= 0.10))}
 
 
-def experiment_extrapolation(reps=3, seed=0, steps=3000, log_ns=range(12, 23, 2), max_elems=1 << 28,
+def experiment_extrapolation(reps=3, seed=0, steps=3000, log_ns=range(11, 23), max_elems=1 << 28,
                              train_fields=(BabyBear, Goldilocks), held_fields=(KoalaBear, TeddyBear),
                              train_batches=(1, 16, 256), held_batch=64, extra_shapes=()):
     """Regret where the model has to extrapolate: trained on full transforms
     of `train_fields` at `train_batches`, evaluated on (a) the same sizes in
     fields it never saw (their field feature is new), (b) a batch size it
     never saw, and (c) `extra_shapes` (e.g. the column encoder's shape),
-    each against exhaustive timing of every class."""
+    each against exhaustive timing of every class.  Every log n of the range
+    is a training size: with every other one (12, 14, .., 22) the held-out
+    regret was 6.9 % interpolated and 5.2 % at batch 64, against 1.1 % and
+    2.1 % on the dense grid (MI355X; repeat timings differ by 0.4 %)."""
     grid = shape_grid(list(train_fields), log_ns, train_batches, partial=False, max_elems=max_elems)
     tr, te = split(grid)  # every 4th shape of the training grid held out: interpolation
     tr_shapes = [grid[i] for i in tr]
@@ -129,7 +132,7 @@ def experiment_extrapolation(reps=3, seed=0, steps=3000, log_ns=range(12, 23, 2)
     X, times = collect(tr_shapes, reps)
     model, loss = train(X, times, seed=seed, steps=steps)
     res = {"train_shapes": len(tr_shapes), "train_fields": [F.name for F in train_fields],
-           "train_batches": list(train_batches), "final_loss": loss}
+           "train_log_n": [min(log_ns), max(log_ns)], "train_batches": list(train_batches), "final_loss": loss}
     for name, shapes in groups.items():
         _, t = collect(shapes, reps)
         held = regrets(model, shapes, t)

@@ -125,17 +125,18 @@ def test_learned_schedules_regret_and_gain():
 
 def test_cost_model_extrapolates():
     """Trained on BabyBear and Goldilocks only, at batches 1, 16 and 256: the
-    model's choices for KoalaBear and TeddyBear (fields it never saw) and for
-    the column encoder's shape (2^22 x 256 columns, beyond the training
-    sizes) stay within 10 % of the exhaustive autotuner's best on average,
-    and no worse than the untuned default on the unseen fields; batch 64
-    (a batch size it never saw) within 25 %."""
+    model's choices on held-out shapes of its grid, for KoalaBear and
+    TeddyBear (fields it never saw), for batch 64 (a batch size it never saw)
+    and for the column encoder's shape (2^22 x 256 columns, beyond the
+    training sizes) stay within 10 % of the exhaustive autotuner's best on
+    average, and no worse than the untuned default (2 points of timing
+    noise allowed) on the held-out groups."""
     enc = (E.BabyBear, 1 << 22, 1 << 22, 256)
     res, model = CM.experiment_extrapolation(reps=3, seed=0, steps=3000, extra_shapes=[enc])
     for k in ("interpolation", "unseen_fields", "unseen_batch_64", "extra"):
         print(k, {kk: vv for kk, vv in res[k].items() if kk != "per_shape"})
-    assert res["unseen_fields"]["mean_regret_model"] <= 0.10
-    assert res["unseen_fields"]["mean_regret_model"] <= res["unseen_fields"]["mean_regret_default"] + 0.02
-    assert res["unseen_batch_64"]["mean_regret_model"] <= 0.25
+    for k in ("interpolation", "unseen_fields", "unseen_batch_64"):
+        assert res[k]["mean_regret_model"] <= 0.10, k
+        assert res[k]["mean_regret_model"] <= res[k]["mean_regret_default"] + 0.02, k
     assert res["extra"]["mean_regret_model"] <= 0.10
     E.tune_clear()
