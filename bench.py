@@ -362,6 +362,18 @@ def _pmc_per_dispatch(run_dir, counter, match):
     return out
 
 
+def _dispatch_names(run_dir, match):
+    """{dispatch_id: kernel name} for the kernels whose name passes `match`."""
+    import glob
+    import sqlite3
+    dbs = glob.glob(os.path.join(run_dir, "**", "*results.db"), recursive=True)
+    if not dbs:
+        raise FileNotFoundError(f"no rocprofv3 results database under {run_dir}")
+    with sqlite3.connect(dbs[0]) as db:
+        return {did: name for did, name in db.execute("select distinct dispatch_id, name from pmc_events")
+                if match(name)}
+
+
 def summarize_profiles(prof_dir, out_path, logn=24):
     """Per-pass HBM traffic and VALU work of the 2^N Goldilocks forward
     transform from five rocprofv3 --pmc runs under prof_dir (copy_fetch,
@@ -411,6 +423,73 @@ def summarize_profiles(prof_dir, out_path, logn=24):
           f"traffic per launch {res['traffic_bytes_per_launch'] / 1e6:.1f} MB; "
           f"VALU lane-ops per element {sum(p['valu_lane_ops_per_elem'] for p in passes):.1f}")
     return res
+
+
+ENC_N, ENC_COLS, ENC_BLOWUP = 1 << 20, 256, 4
+ENC_REPS = 4  # encodes of an --encoder-only run
+
+
+def encoder_only(E, torch, reps):
+    """`reps` encodes of the 256 x 2^20 BabyBear blowup-4 column batch on the
+    shipped tuned schedule, nothing else: the run a counter profile of the
+    encoder's kernels is taken from (--summarize-encoder)."""
+    F = E.BabyBear
+    m = ENC_N * ENC_BLOWUP
+    coeffs = torch.randint(0, F.p, (ENC_N * ENC_COLS,), dtype=torch.int32, device="cuda")
+    evals = torch.empty(m * ENC_COLS, dtype=torch.int32, device="cuda")
+    for _ in range(reps):
+        E.coset_lde_device(coeffs, evals, F, ENC_N, ENC_COLS, ENC_BLOWUP)
+    torch.cuda.synchronize()
+    print(json.dumps({"encoder_only": {"encodes": reps, "columns": ENC_COLS, "n": ENC_N, "blowup": ENC_BLOWUP}}),
+          flush=True)
+
+
+def summarize_encoder(prof_dir):
+    """HBM bytes per column of the encoder (`bench.py --encoder-only`) from
+    rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs under prof_dir (fetch_enc,
+    write_enc), scaled by the copy calibration runs (copy_fetch, copy_write),
+    per pass kernel; the first encode is skipped.  Markdown on stdout."""
+    copy_bytes = (1 << 24) * 8
+
+    def is_copy(name):
+        return "copy" in name.lower()
+
+    def bb_pass(name):  # the encoder's pass kernels: BabyBear instantiations
+        return "k_stockham_pass<omr::Mont32<2013265921u" in name
+    cf = _pmc_per_dispatch(os.path.join(prof_dir, "copy_fetch"), "FETCH_SIZE", is_copy)
+    cw = _pmc_per_dispatch(os.path.join(prof_dir, "copy_write"), "WRITE_SIZE", is_copy)
+    fetch_scale = copy_bytes / 1024 / (sum(cf.values()) / len(cf))
+    write_scale = copy_bytes / 1024 / (sum(cw.values()) / len(cw))
+    f = _pmc_per_dispatch(os.path.join(prof_dir, "fetch_enc"), "FETCH_SIZE", bb_pass)
+    w = _pmc_per_dispatch(os.path.join(prof_dir, "write_enc"), "WRITE_SIZE", bb_pass)
+    names = _dispatch_names(os.path.join(prof_dir, "fetch_enc"), bb_pass)
+    ids = sorted(f)
+    npe = len(ids) // ENC_REPS  # pass launches per encode (every column group's passes)
+    ids_w = sorted(w)
+    first = [names[i] for i in ids[npe:2 * npe]]
+    npass = next((k for k in range(1, npe) if first[k] == first[0]), npe)  # passes per column group
+    rows = [{"name": first[k], "fetch": 0.0, "write": 0.0} for k in range(npass)]
+    for k, (i, iw) in enumerate(zip(ids[npe:], ids_w[npe:])):
+        r = rows[(k % npe) % npass]
+        r["fetch"] += f[i] * 1024 * fetch_scale
+        r["write"] += w[iw] * 1024 * write_scale
+    encodes = ENC_REPS - 1
+    alg = (ENC_N + ENC_N * ENC_BLOWUP) * 4
+    print(f"{npe // npass} column groups of {npass} pass launches per encode\n")
+    print("| pass | kernel | HBM read per column (MiB) | HBM write per column (MiB) |")
+    print("|---|---|---|---|")
+    tot_r = tot_w = 0.0
+    for k, r in enumerate(rows):
+        rd = r["fetch"] / encodes / ENC_COLS
+        wr = r["write"] / encodes / ENC_COLS
+        tot_r += rd
+        tot_w += wr
+        nm = r["name"].split("(")[0].replace("omr::Mont32<2013265921u, 31u, 27, 0>", "BabyBear")
+        print(f"| {k} | `{nm}` | {rd / 2**20:.2f} | {wr / 2**20:.2f} |")
+    print(f"\nper column: read {tot_r / 2**20:.2f} MiB + write {tot_w / 2**20:.2f} MiB = "
+          f"{(tot_r + tot_w) / 2**20:.2f} MiB against {alg / 2**20:.0f} MiB algorithmic "
+          f"(read 2^20 + write 2^22 words of 4 bytes): {(tot_r + tot_w) / alg:.2f}x; "
+          f"scale factors FETCH_SIZE x{fetch_scale:.3f}, WRITE_SIZE x{write_scale:.3f}")
 
 
 def summarize_sq(sq_dir, clk_dir, logn=24):
@@ -522,6 +601,12 @@ def main():
     ap.add_argument("--profile-out", default=os.path.join(ROOT, "profiles", "traffic_k_stockham_pass_goldilocks_2p24.json"))
     ap.add_argument("--summarize-sq", nargs=2, metavar=("SQ_DIR", "CLOCK_DIR"),
                     help="print per-pass SQ counters and clock from two rocprofv3 --pmc runs of this bench")
+    ap.add_argument("--encoder-only", action="store_true",
+                    help=f"only run {ENC_REPS} encodes of the 256 x 2^20 BabyBear blowup-4 columns (tuned schedule), "
+                         "for a counter profile of the encoder")
+    ap.add_argument("--summarize-encoder", metavar="DIR",
+                    help="no GPU work: HBM bytes per column of the encoder from rocprofv3 --pmc runs under DIR "
+                         "(copy_fetch, copy_write, fetch_enc, write_enc)")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="no GPU work: print the kernel-duration table of a rocprofv3 --kernel-trace run under DIR")
     args = ap.parse_args()
@@ -538,6 +623,9 @@ def main():
         return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace, args.logn)
+        return
+    if args.summarize_encoder:
+        summarize_encoder(args.summarize_encoder)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path "
@@ -558,6 +646,10 @@ def main():
         return
 
     import omr_ntt as E
+
+    if args.encoder_only:
+        encoder_only(E, torch, ENC_REPS)
+        return
 
     F = E.Goldilocks
     n = 1 << args.logn
