@@ -526,6 +526,7 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) try {
   free_buf(ctx->io[1]);
   free_buf(ctx->part);
   free_buf(ctx->part2);
+  free_buf(ctx->counters);
   for (auto &b : ctx->hio) free_buf(b);
   for (auto &b : ctx->words) free_buf(b);
   free_stage(ctx->stage);

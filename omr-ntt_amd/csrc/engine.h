@@ -437,10 +437,12 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
       const bool one = logNs <= a.log_mid;
       // Rows k = t0 + m RE: w^(k p) = A B^m with A = w^(t0 p), B = w^(RE p), in
       // two chains (even / odd m) stepped by B^2: two table lookups per
-      // thread instead of one or two per element.  Used where it saves loads
-      // at no extra products (the two-level case), and for 32-bit fields
-      // everywhere (their products cost 6 VALU slots, a lookup more).
-      if (sizeof(T) == 4 || !one) {
+      // thread instead of one or two per element.  For Goldilocks that costs
+      // a second product per element against one lookup per element where a
+      // single table covers N/s, and still measured faster (2^24 [8,8,8]:
+      // 203.5 -> 201.5 us): the per-element gathers of the middle pass
+      // waited longer than the products took.
+      {
         const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
         const bool small = logNs <= a.log_rmax;
         const T *tab = small ? pyr + (1u << logNs) : (const T *)a.tw_mid;
@@ -466,15 +468,6 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
             x[m + r] = F::mulw(x[m + r], ch[r]);
             if (m + 2 < E) ch[r] = F::wmul(ch[r], B2);
           }
-        }
-      } else {  // Goldilocks, one lookup: w_(N/s)^e from the root pyramid or the mid table
-        const bool small = logNs <= a.log_rmax;
-        const T *tab = small ? pyr + (1u << logNs) : (const T *)a.tw_mid;
-        const uint32_t sh = small ? 0u : a.log_mid - logNs;
-#pragma unroll
-        for (int m = 0; m < E; m++) {
-          const uint32_t e = (t0 + m * RE) * p;
-          x[m] = F::mulw(x[m], tab[e << sh]);  // e = 0 multiplies by 1: no divergent branch
         }
       }
     }

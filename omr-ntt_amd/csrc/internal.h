@@ -88,6 +88,7 @@ struct omr_ctx {
   omr::DevBuf scratch[3];
   omr::DevBuf io[2];
   omr::DevBuf part, part2;
+  omr::DevBuf counters;  // k_stride_fused's "last one done" counters, zeroed at allocation
   omr::DevBuf words[2];  // narrowed input / output of the word-width entry points
   // staging of the host-buffer forms (omr_intt, omr_ntt_batched,
   // omr_coset_lde, omr_coset_decode), under their own lock: they call the
@@ -208,6 +209,18 @@ omr_status launch_td_recombine(int f, const void *Y, void *part, uint64_t out_le
                                hipStream_t s);
 omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m, uint32_t nchunks,
                        hipStream_t s, uint32_t nvec = 1, uint64_t xs = 0, uint64_t ps = 0);
+// One-launch stride subset (k_stride_fused) of m outputs of an N-element
+// vector of es-byte words: grid workgroups of rows_per_wg rows, first-level
+// groups of `group` workgroups.  stride_fused_plan returns false where the
+// form does not apply (it needs 16 <= m <= 512, m < N and a row per row
+// subset of a workgroup).  part holds grid x m words, part2 grid / group x m,
+// cnt grid / group + 1 counters that are zero on entry and on exit.
+struct StrideFusedPlan {
+  uint32_t grid = 0, group = 0, rows_per_wg = 0, logm = 0;
+};
+bool stride_fused_plan(int es, uint64_t N, uint64_t m, StrideFusedPlan *plan);
+omr_status launch_stride_fused(int f, const void *x, void *part, void *part2, uint32_t *cnt, void *out,
+                               const void *root_m, const StrideFusedPlan &plan, hipStream_t s);
 // out[v os + k] = sum_{t < n} part[(v m + k) n + t], k < m (<= 2^31), v < nvec (<= 65535)
 omr_status launch_sum_contig(int f, const void *part, void *out, uint64_t n, uint64_t m, uint32_t nvec,
                              uint64_t os, hipStream_t s);

@@ -155,6 +155,158 @@ __global__ void __launch_bounds__(256) k_fold(const typename F::T *__restrict__ 
   part[(uint64_t)blockIdx.y * m + i] = acc;
 }
 
+// Stride subset out[j] = X[j N/m] of one vector in one launch, for
+// 16 <= m <= 512 (powers of two).  X[j N/m] = sum_i y[i] w_m^(ij) with the
+// fold y[i] = sum_t x[i + m t], so:
+// * workgroup b of G adds rows [b R, (b+1) R) of x viewed as N/m rows of m
+//   (one contiguous block of R m elements; 16-byte loads, kSfUnroll rows in
+//   flight per thread; the 512 threads are m/V column groups of V = 16 bytes
+//   x 512 V/m row subsets, added up through LDS) into part[b m + i];
+// * the workgroups finish in two levels of "last one done": the last of
+//   each group of `group` workgroups (counter cnt[g]) adds its group's rows
+//   into part2[g m + i]; the last of those (counter cnt[G / group]) adds
+//   the part2 rows, runs the m-point NTT in LDS (radix-2 decimation in
+//   frequency over the roots w_m^k, level log m of the root pyramid) and
+//   writes out[j] in natural order, then clears the counters for the next
+//   launch.  No workgroup waits for another: the ones that are not last
+//   simply end.
+constexpr int kSfThreads = 512;
+constexpr int kSfUnroll = 8;
+constexpr int kSfMaxLogM = 9;
+constexpr int kSfGroup = 16;      // workgroups per first-level group
+constexpr int kSfMaxGroups = 32;  // 512 workgroups at most
+
+// Partial sums that one workgroup writes and another reads within the
+// launch go through agent-scope atomic stores and loads (coherent across
+// the XCDs' L2s without cache maintenance), ordered around the counters by
+// workgroup-scope fences (which wait for the stores to complete).  Plain
+// stores with agent-scope release / acquire fences were bit-exact too but
+// write back and invalidate an XCD's L2 per workgroup: 83 / 110 us against
+// 12.6 / 16.6 us (BabyBear / Goldilocks, N = 2^22, m = 256).
+template <class T>
+__device__ __forceinline__ void sf_store(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T sf_load(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void sf_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+__device__ __forceinline__ void sf_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+
+// Column sums of nrows rows of m (<= kSfThreads) partials written by other
+// workgroups of this launch: thread (i, sub) = (tid mod m, tid div m) loads
+// rows sub, sub + 512/m, ... all at once (at most MAXR), the row subsets
+// meet in LDS; returns column tid's sum in threads tid < m.
+template <class F, int MAXR>
+__device__ __forceinline__ typename F::T sf_sum_rows(const typename F::T *src, uint32_t nrows, uint32_t m,
+                                                     uint32_t tid, typename F::T *lds) {
+  using T = typename F::T;
+  const uint32_t cs = kSfThreads / m;
+  const uint32_t i = tid % m, sub = tid / m;
+  T v[MAXR];
+#pragma unroll
+  for (int k = 0; k < MAXR; k++) {
+    const uint32_t r = sub + (uint32_t)k * cs;
+    v[k] = r < nrows ? sf_load(&src[(uint64_t)r * m + i]) : (T)0;
+  }
+  T s = 0;
+#pragma unroll
+  for (int k = 0; k < MAXR; k++) s = F::add(s, v[k]);
+  __syncthreads();  // lds is free
+  lds[sub * m + i] = s;
+  __syncthreads();
+  T t = 0;
+  if (tid < m)
+    for (uint32_t u = 0; u < cs; u++) t = F::add(t, lds[u * m + tid]);
+  return t;
+}
+
+template <class F>
+__global__ void __launch_bounds__(kSfThreads) k_stride_fused(const typename F::T *__restrict__ x,
+                                                             typename F::T *__restrict__ part,
+                                                             typename F::T *__restrict__ part2, uint32_t *cnt,
+                                                             typename F::T *__restrict__ out,
+                                                             const typename F::T *__restrict__ root_m,
+                                                             uint32_t logm, uint32_t rows_per_wg, uint32_t group) {
+  using T = typename F::T;
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T VT __attribute__((ext_vector_type(V)));
+  __shared__ T lds[kSfThreads * V];
+  __shared__ uint32_t ticket;
+  const uint32_t m = 1u << logm;
+  const uint32_t tpr = m / V;              // threads per row
+  const uint32_t rsub = kSfThreads / tpr;  // row subsets
+  const uint32_t tid = threadIdx.x;
+  const uint32_t cg = tid % tpr, rs = tid / tpr;
+
+  {  // fold of this workgroup's rows: thread (cg, rs) takes rows rs + k rsub
+    const T *base = x + (uint64_t)blockIdx.x * rows_per_wg * m + (uint64_t)cg * V;
+    const uint32_t K = rows_per_wg / rsub;
+    T acc[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) acc[v] = 0;
+    for (uint32_t k0 = 0; k0 < K; k0 += kSfUnroll) {
+      VT w[kSfUnroll];
+#pragma unroll
+      for (int u = 0; u < kSfUnroll; u++)
+        if (k0 + u < K)
+          w[u] = __builtin_nontemporal_load((const VT *)(base + (uint64_t)(rs + (k0 + u) * rsub) * m));
+#pragma unroll
+      for (int u = 0; u < kSfUnroll; u++)
+        if (k0 + u < K) {
+#pragma unroll
+          for (int v = 0; v < V; v++) acc[v] = F::add(acc[v], w[u][v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) lds[rs * m + cg * V + v] = acc[v];
+  }
+  __syncthreads();
+  if (tid < m) {
+    T s = 0;
+    for (uint32_t r = 0; r < rsub; r++) s = F::add(s, lds[r * m + tid]);
+    sf_store(&part[(uint64_t)blockIdx.x * m + tid], s);
+  }
+
+  const uint32_t g = blockIdx.x / group, ngroups = gridDim.x / group;
+  sf_release();
+  __syncthreads();
+  if (tid == 0) ticket = __hip_atomic_fetch_add(&cnt[g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (ticket != group - 1) return;  // not the last of its group
+  sf_acquire();
+  {
+    const T s = sf_sum_rows<F, kSfGroup>(part + (uint64_t)g * group * m, group, m, tid, lds);
+    if (tid < m) sf_store(&part2[(uint64_t)g * m + tid], s);
+  }
+  sf_release();
+  __syncthreads();
+  if (tid == 0) ticket = __hip_atomic_fetch_add(&cnt[ngroups], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (ticket != ngroups - 1) return;  // not the last group
+  sf_acquire();
+  {
+    const T s = sf_sum_rows<F, kSfMaxGroups>(part2, ngroups, m, tid, lds);
+    __syncthreads();  // lds is the NTT's now
+    if (tid < m) lds[tid] = s;
+  }
+  if (tid <= ngroups) cnt[tid] = 0;  // every workgroup has counted: clear for the next launch
+  __syncthreads();
+  for (uint32_t len = m; len >= 2; len >>= 1) {  // y -> NTT_m(y), bit-reversed order
+    const uint32_t half = len >> 1;
+    if (tid < m / 2) {
+      const uint32_t j = tid & (half - 1);
+      const uint32_t i0 = (tid - j) * 2 + j;  // block (tid / half) of length len, offset j
+      const T u = lds[i0], v = lds[i0 + half];
+      lds[i0] = F::add(u, v);
+      lds[i0 + half] = F::mulw(F::sub(u, v), root_m[j * (m / len)]);  // j = 0: times root_m[0] = 1
+    }
+    __syncthreads();
+  }
+  if (tid < m) out[tid] = lds[__builtin_bitreverse32(tid) >> (32 - logm)];
+}
+
 // Contiguous row sums: out[v os + k] = sum_{t < n} part[(v m + k) n + t],
 // grid (m, nvec), one 256-thread block per sum (coalesced loads, LDS tree).
 template <class F>

@@ -107,6 +107,30 @@ omr_status launch_fold(int f, const void *x, void *part, uint64_t n, uint64_t m,
   return last_launch();
 }
 
+bool stride_fused_plan(int es, uint64_t N, uint64_t m, StrideFusedPlan *plan) {
+  if (m < 16 || m > (1ull << kSfMaxLogM) || m >= N || (m & (m - 1)) || (N & (N - 1))) return false;
+  const uint64_t tpr = m / (16 / es);  // threads per row (16-byte loads)
+  const uint64_t rsub = kSfThreads / tpr;
+  const uint64_t rows = N / m;
+  if (rows < rsub) return false;
+  // powers of two throughout: rows_per_wg is a multiple of rsub, grid of group
+  plan->grid = (uint32_t)std::min<uint64_t>((uint64_t)kSfGroup * kSfMaxGroups, rows / rsub);
+  plan->group = std::min<uint32_t>(kSfGroup, plan->grid);
+  plan->rows_per_wg = (uint32_t)(rows / plan->grid);
+  plan->logm = (uint32_t)__builtin_ctzll(m);
+  return true;
+}
+
+omr_status launch_stride_fused(int f, const void *x, void *part, void *part2, uint32_t *cnt, void *out,
+                               const void *root_m, const StrideFusedPlan &plan, hipStream_t s) {
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_stride_fused<F>, dim3(plan.grid), dim3(kSfThreads), 0, s, (const T *)x, (T *)part,
+                       (T *)part2, cnt, (T *)out, (const T *)root_m, plan.logm, plan.rows_per_wg, plan.group);
+  });
+  return last_launch();
+}
+
 omr_status launch_sum_contig(int f, const void *part, void *out, uint64_t n, uint64_t m, uint32_t nvec,
                              uint64_t os, hipStream_t s) {
   OMR_FIELD_SWITCH(f, {

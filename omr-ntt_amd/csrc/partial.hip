@@ -217,6 +217,23 @@ uint32_t multipass_log_n1(uint32_t logN, uint64_t out_len) {
   return (l >= lo && l > (uint32_t)kPrefixMaxLogR) ? l : 0;
 }
 
+static omr_status stride_fused(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN, uint64_t omega,
+                               uint64_t m, const StrideFusedPlan &plan, hipStream_t stream) {
+  const int es = field_bytes(f);
+  const TwTables *tab;
+  omr_status st = get_tables(ctx, f, logN, omega, &tab);
+  if (st != OMR_OK) return st;
+  if ((st = ensure_buf(ctx->part, (size_t)plan.grid * m * es)) != OMR_OK) return st;
+  if ((st = ensure_buf(ctx->part2, (size_t)(plan.grid / plan.group) * m * es)) != OMR_OK) return st;
+  if (!ctx->counters.p) {
+    if ((st = ensure_buf(ctx->counters, 4096)) != OMR_OK) return st;
+    if (hipMemset(ctx->counters.p, 0, 4096) != hipSuccess) return OMR_E_DEVICE;
+  }
+  const void *root_m = (const char *)tab->pyr + m * es;  // level log m of the root pyramid: w_m^k, k < m
+  return launch_stride_fused(f, d_in, ctx->part.p, ctx->part2.p, (uint32_t *)ctx->counters.p, d_out, root_m, plan,
+                             stream);
+}
+
 omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint32_t logN,
                         uint64_t omega, uint64_t out_len, int mode, int method, uint32_t log_n1,
                         hipStream_t stream) {
@@ -233,6 +250,10 @@ omr_status partial_impl(omr_ctx *ctx, int f, const void *d_in, void *d_out, uint
       default_schedule(f, logN, sch);
       return run_transform(ctx, f, d_in, d_out, logN, omega, sch, 1, N, N, o, stream);
     }
+    // (16-byte loads: an input aligned to 16 bytes)
+    StrideFusedPlan sfp;
+    if ((uintptr_t)d_in % 16 == 0 && stride_fused_plan(es, N, m, &sfp))
+      return stride_fused(ctx, f, d_in, d_out, logN, omega, m, sfp, stream);
     // fold x viewed as N/m rows of m into scratch[2]
     if ((st = ensure_buf(ctx->scratch[2], (size_t)m * es)) != OMR_OK) return st;
     if ((st = fold_rows(ctx, f, d_in, N / m, m, 1, 0, ctx->scratch[2].p, 0, stream)) != OMR_OK) return st;

@@ -877,3 +877,32 @@ def test_concurrent_host_threads_share_one_context():
         th.join(timeout=120)
     assert not any(th.is_alive() for th in threads), "a worker did not finish"
     assert not errors, errors
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_stride_subset_one_launch(fname):
+    """Stride subsets of 16 <= m <= 512 outputs run as one kernel: folds over
+    up to 512 workgroups finished by two levels of last-workgroup-done
+    counters and an m-point NTT in LDS.  Every m around that range (8 and
+    1024 take the multi-launch path), vectors too short for a full row subset
+    (fallback), calls back to back with different m (the counters are cleared
+    by each launch), and in place (d_in == d_out)."""
+    F = field(fname)
+    for logn in (8, 10, 14, 18, 22):
+        n = 1 << logn
+        x = inputs(F, n, 700 + logn)
+        X = O.ntt(F.id, x)
+        t = E.to_device(x, F)
+        for logm in range(3, 11):
+            m = 1 << logm
+            if m > n:
+                continue
+            print(f"case: stride subset {F} 2^{logn} m={m}", flush=True)
+            out = E.empty_device(m, F)
+            for rep in range(2):
+                E.partial_device(t, out, F, m, mode=E.STRIDE)
+                assert np.array_equal(E.from_device(out, F), X[:: n // m]), (logn, m, rep)
+        if n >= 1 << 10:
+            buf = t.clone()
+            E.partial_device(buf, buf, F, 256, mode=E.STRIDE)
+            assert np.array_equal(E.from_device(buf, F)[:256], X[:: n // 256]), (logn, "in place")
