@@ -415,7 +415,16 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     int logE = sch.logE;
     if (!has_pass_kernel(logR, logE, es)) logE = auto_log_e(logR);
     if (!has_pass_kernel(logR, logE, es)) return OMR_E_SCHEDULE;
-    const int logC = pass_log_cols((int)logN, logR, logE, es);
+    // Small jobs (batch x N words) take narrower tiles than the coalescing
+    // rule gives, so that their few tiles spread over more CUs: a 2^16-word
+    // vector has 16 tiles of radix 256 at full width.  Kernel time per
+    // transform, MI355X: BabyBear 2^16 [8,8] 10.7 -> 8.2 us at 4 columns;
+    // Goldilocks 2^16 13.7 -> 12.3 us at 8, 2^18 16.5 -> 14.8 us at 16;
+    // from 2^18 (BabyBear) / 2^20 (Goldilocks) on, full width is faster.
+    int logC = pass_log_cols((int)logN, logR, logE, es);
+    const uint64_t job = (uint64_t)batch << logN;
+    if (job <= (1ull << 16)) logC = std::min(logC, es == 4 ? 2 : 3);
+    else if (es == 8 && job <= (1ull << 18)) logC = std::min(logC, 4);
     PassArgs a{};
     a.src = X[i];
     a.dst = (void *)X[i + 1];
