@@ -536,6 +536,10 @@ extern "C" omr_status omr_ctx_destroy(omr_ctx *ctx) try {
   free_buf(ctx->part);
   free_buf(ctx->part2);
   free_buf(ctx->counters);
+  for (auto &b : ctx->side_scratch) free_buf(b);
+  if (ctx->side_fork) (void)hipEventDestroy(ctx->side_fork);
+  if (ctx->side_join) (void)hipEventDestroy(ctx->side_join);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   for (auto &b : ctx->hio) free_buf(b);
   for (auto &b : ctx->words) free_buf(b);
   free_stage(ctx->stage);

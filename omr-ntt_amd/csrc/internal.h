@@ -89,6 +89,12 @@ struct omr_ctx {
   omr::DevBuf io[2];
   omr::DevBuf part, part2;
   omr::DevBuf counters;  // k_stride_fused's "last one done" counters, zeroed at allocation
+  // second stream of the column batches (coset LDE / decode): created on
+  // first use, forked from and joined back to the caller's stream within a
+  // call, with its own intermediates
+  hipStream_t side = nullptr;
+  hipEvent_t side_fork = nullptr, side_join = nullptr;
+  omr::DevBuf side_scratch[2];
   omr::DevBuf words[2];  // narrowed input / output of the word-width entry points
   // staging of the host-buffer forms (omr_intt, omr_ntt_batched,
   // omr_coset_lde, omr_coset_decode), under their own lock: they call the

@@ -906,3 +906,37 @@ def test_stride_subset_one_launch(fname):
             buf = t.clone()
             E.partial_device(buf, buf, F, 256, mode=E.STRIDE)
             assert np.array_equal(E.from_device(buf, F)[:256], X[:: n // 256]), (logn, "in place")
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_coset_lde_columns_two_streams(fname):
+    """Column batches larger than one group of ~128 MiB of intermediates run
+    their groups on two streams (the caller's and a second one with its own
+    intermediates).  2^20 -> 2^22 columns, two more columns than one group
+    holds: sampled columns of both streams against the oracle, the round
+    trip, and a corrupted evaluation on either stream caught by decode's
+    nonzero count."""
+    F = field(fname)
+    n, blow = 1 << 20, 4
+    group = (128 << 20) // (n * blow * F.bytes)
+    ncols = group + 2
+    print(f"case: two-stream LDE {F} {ncols} columns of 2^20 -> 2^22 (group {group})", flush=True)
+    x = inputs(F, n * ncols, 77)
+    c = E.to_device(x, F)
+    ev = E.empty_device(n * blow * ncols, F)
+    E.coset_lde_device(c, ev, F, n, ncols, blow)
+    for col in (0, group - 1, group, ncols - 1):  # first stream, then the second
+        got = E.from_device(ev[col * n * blow:(col + 1) * n * blow], F)
+        assert np.array_equal(got, O.coset_lde(F.id, x[col * n:(col + 1) * n], blow)), col
+    back = E.empty_device(n * ncols, F)
+    nz = torch.zeros(1, dtype=torch.int32, device="cuda")
+    E.coset_decode_device(ev, back, F, n, ncols, blow, nonzero=nz)
+    assert torch.equal(back, c) and int(nz.item()) == 0
+    for col in (1, ncols - 1):
+        bad = ev.clone()
+        i = col * n * blow + 12345
+        bad[i] = (bad[i] + 1) % F.p if F.bytes == 4 else bad[i] ^ 1
+        E.coset_decode_device(bad, back, F, n, ncols, blow, nonzero=nz)
+        assert int(nz.item()) > 0, col
+        del bad
+    torch.cuda.empty_cache()
