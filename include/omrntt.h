@@ -95,6 +95,17 @@ omr_status omr_field_info(int field, uint64_t *modulus, uint64_t *generator, int
 int omr_field_bytes(int field);
 /* NttDomain::new checks plus F::get_root_of_unity(N) (src/encoder.rs:23-35). */
 omr_status omr_root_of_unity(int field, uint64_t n, uint64_t *omega);
+/* NttDomain's public tables (src/encoder.rs:14-18, built at :37-44):
+ * twiddles[k] = omega^k for k < n and bitrev_twiddles[i] =
+ * omega^bitrev(i, log2 n - 1) for i < n/2, canonical words.  The engine's
+ * transforms never need them (it builds its own compact tables from omega);
+ * they are for hosts that read the fields.  Either output may be NULL.
+ * Device form: d_* are device buffers of n and n/2 words, filled on
+ * `stream`; host form: host buffers, synchronous. */
+omr_status omr_domain_tables_device(omr_ctx *ctx, int field, uint64_t n, uint64_t omega, void *d_twiddles,
+                                    void *d_bitrev_twiddles, void *stream);
+omr_status omr_domain_tables(omr_ctx *ctx, int field, uint64_t n, uint64_t omega, void *twiddles,
+                             void *bitrev_twiddles);
 /* NttEncoder::name (src/encoder.rs:73); NULL for an unknown variant. */
 const char *omr_variant_name(int variant);
 /* Whether `variant` accepts N = 2^log_n (registry rule, src/encoders/mod.rs:28-46). */

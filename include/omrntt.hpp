@@ -68,7 +68,8 @@ inline omr_ctx *context() {
 }
 
 // NttDomain::new (src/encoder.rs:23-52): N, log_N and omega.  The device
-// derives its own twiddle tables from omega.
+// derives its own twiddle tables from omega; the reference's public tables
+// twiddles / bitrev_twiddles (src/encoder.rs:14-18) are built on request.
 template <class F>
 struct NttDomain {
   uint64_t N;
@@ -79,6 +80,19 @@ struct NttDomain {
     while ((1ull << log_N) < n) log_N++;
   }
   NttDomain(uint64_t n, uint64_t w) : NttDomain(n) { omega = w; }
+  // twiddles[k] = omega^k, k < N
+  std::vector<typename F::T> twiddles() const {
+    std::vector<typename F::T> t(N);
+    check(omr_domain_tables(context(), F::id, N, omega, t.data(), nullptr), "NttDomain::twiddles");
+    return t;
+  }
+  // bitrev_twiddles[i] = omega^bitrev(i, log_N - 1), i < N/2
+  std::vector<typename F::T> bitrev_twiddles() const {
+    std::vector<typename F::T> t(N / 2);
+    check(omr_domain_tables(context(), F::id, N, omega, nullptr, t.empty() ? nullptr : t.data()),
+          "NttDomain::bitrev_twiddles");
+    return t;
+  }
 };
 
 // trait NttEncoder<F> (src/encoder.rs:70-74).

@@ -1061,3 +1061,54 @@ extern "C" omr_status omr_coset_decode(omr_ctx *ctx, int field, const void *eval
 } catch (...) {
   return OMR_E_DEVICE;
 }
+
+// NttDomain's public twiddles / bitrev_twiddles (src/encoder.rs:14-18).
+extern "C" omr_status omr_domain_tables_device(omr_ctx *ctx, int field, uint64_t n, uint64_t omega,
+                                               void *d_twiddles, void *d_bitrev_twiddles, void *stream) try {
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (!ctx) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  hipStream_t hs = (hipStream_t)stream;
+  StreamOrder ord(ctx, hs);
+  uint32_t logN = 0;
+  while ((1ull << logN) < n) logN++;
+  if (n == 1) {  // twiddles = [1], bitrev_twiddles = []
+    if (!d_twiddles) return OMR_OK;
+    static const uint64_t one = 1;  // canonical 1, little-endian: the low word serves 4-byte fields
+    return hipMemcpyAsync(d_twiddles, &one, field_bytes(field), hipMemcpyHostToDevice, hs) == hipSuccess
+               ? OMR_OK
+               : OMR_E_DEVICE;
+  }
+  const TwTables *tab;
+  if ((st = get_tables(ctx, field, logN, omega, &tab)) != OMR_OK) return st;
+  return launch_domain_tables(field, d_twiddles, d_bitrev_twiddles, n, logN, tab, hs);
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
+}
+
+extern "C" omr_status omr_domain_tables(omr_ctx *ctx, int field, uint64_t n, uint64_t omega, void *twiddles,
+                                        void *bitrev_twiddles) try {
+  omr_status st = validate_domain(field, n, omega);
+  if (st != OMR_OK) return st;
+  if (!ctx) return OMR_E_NULL;
+  std::lock_guard<std::mutex> lk(ctx->host_mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return OMR_E_DEVICE;
+  const size_t es = field_bytes(field);
+  if (twiddles && (st = ensure_buf(ctx->hio[0], n * es)) != OMR_OK) return st;
+  if (bitrev_twiddles && n >= 2 && (st = ensure_buf(ctx->hio[1], n / 2 * es)) != OMR_OK) return st;
+  if ((st = omr_domain_tables_device(ctx, field, n, omega, twiddles ? ctx->hio[0].p : nullptr,
+                                     (bitrev_twiddles && n >= 2) ? ctx->hio[1].p : nullptr, nullptr)) != OMR_OK)
+    return st;
+  if (twiddles && (st = device_to_host(ctx, twiddles, ctx->hio[0].p, n * es)) != OMR_OK) return st;
+  if (bitrev_twiddles && n >= 2 && (st = device_to_host(ctx, bitrev_twiddles, ctx->hio[1].p, n / 2 * es)) != OMR_OK)
+    return st;
+  return OMR_OK;
+} catch (const std::bad_alloc &) {
+  return OMR_E_ALLOC;
+} catch (...) {
+  return OMR_E_DEVICE;
+}

@@ -231,6 +231,10 @@ omr_status launch_stride_fused(int f, const void *x, void *part, void *part2, ui
 omr_status launch_sum_contig(int f, const void *part, void *out, uint64_t n, uint64_t m, uint32_t nvec,
                              uint64_t os, hipStream_t s);
 omr_status launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+// NttDomain's twiddles (n words) and bitrev_twiddles (n/2 words) from the
+// tables of (log n, w); either output may be null
+omr_status launch_domain_tables(int f, void *tw, void *brev, uint64_t n, uint32_t logN, const TwTables *t,
+                                hipStream_t s);
 // 64-bit words <-> 32-bit values: out32[i] = low half of in64[i] / out64[i] = in32[i]
 omr_status launch_narrow(const uint64_t *in, uint32_t *out, uint64_t n, hipStream_t s);
 omr_status launch_widen(const uint32_t *in, uint64_t *out, uint64_t n, hipStream_t s);

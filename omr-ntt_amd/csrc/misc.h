@@ -347,6 +347,24 @@ __global__ void __launch_bounds__(256) k_sum_short(const typename F::T *__restri
   out[(uint64_t)blockIdx.y * os + k] = F::add(F::add(acc[0], acc[1]), F::add(acc[2], acc[3]));
 }
 
+// NttDomain's public tables (src/encoder.rs:14-18, built at :37-44):
+// twiddles[k] = w^k for k < n and bitrev_twiddles[i] = w^bitrev(i, log n - 1)
+// for i < n/2 (bitrev as src/encoder.rs:54-57), canonical, from the
+// engine's two-level table of w; either output may be null.  One thread
+// per k.
+template <class F>
+__global__ void __launch_bounds__(256) k_domain_tables(typename F::T *__restrict__ tw, typename F::T *__restrict__ brev,
+                                                       uint64_t n, uint32_t logN, Tw2 t) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (tw) tw[k] = F::from_w(tw2<F>(t, k));
+  if (brev && k < n / 2) {
+    const uint32_t lh = logN - 1;  // n >= 2 here
+    const uint64_t e = lh ? (__builtin_bitreverse64(k) >> (64 - lh)) : 0;
+    brev[k] = F::from_w(tw2<F>(t, e));
+  }
+}
+
 // Elementwise field ops on canonical operands (device arithmetic unit test):
 // op 0 add, 1 sub, 2 mul, 3 to_w then from_w round trip of a, 4 mulw(a, to_w(b)).
 template <class F>

@@ -145,6 +145,17 @@ omr_status launch_sum_contig(int f, const void *part, void *out, uint64_t n, uin
   return last_launch();
 }
 
+omr_status launch_domain_tables(int f, void *tw, void *brev, uint64_t n, uint32_t logN, const TwTables *t,
+                                hipStream_t s) {
+  if (n == 0) return OMR_OK;
+  OMR_FIELD_SWITCH(f, {
+    using T = typename F::T;
+    hipLaunchKernelGGL(k_domain_tables<F>, dim3(blocks(n, 256)), dim3(256), 0, s, (T *)tw, (T *)brev, n, logN,
+                       tw2_of(t));
+  });
+  return last_launch();
+}
+
 omr_status launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
   if (n == 0) return OMR_OK;
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n, 256)), dim3(256), 0, s, p, n, v);

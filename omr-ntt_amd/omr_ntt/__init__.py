@@ -91,6 +91,8 @@ def lib():
             "omr_field_info": (i32, [i32, P64, P64, Pi]),
             "omr_field_bytes": (i32, [i32]),
             "omr_root_of_unity": (i32, [i32, u64, P64]),
+            "omr_domain_tables": (i32, [_VP, i32, u64, u64, _VP, _VP]),
+            "omr_domain_tables_device": (i32, [_VP, i32, u64, u64, _VP, _VP, _VP]),
             "omr_variant_name": (ctypes.c_char_p, [i32]),
             "omr_variant_supports": (i32, [i32, u32]),
             "omr_ntt": (i32, [_VP, i32, i32, _VP, u64, u64, u64]),
@@ -195,9 +197,10 @@ def default_schedule(field, n):
 
 class NttDomain:
     """Reusable NTT parameters: N, log_N and the primitive N-th root omega
-    (F::get_root_of_unity(N)).  The reference also materialises twiddles /
-    bitrev_twiddles on the host; the engine derives its own device tables from
-    omega, so they are computed only on request."""
+    (F::get_root_of_unity(N)), plus the reference's public tables twiddles
+    (omega^k, k < N) and bitrev_twiddles (omega^bitrev(i, log_N - 1),
+    i < N/2).  The engine derives its own device tables from omega, so the
+    two tables are built (on the GPU) only when read."""
 
     def __init__(self, N, field=DefaultField, omega=None):
         if N <= 0 or N & (N - 1):
@@ -207,15 +210,40 @@ class NttDomain:
             raise OmrError(3, f"N={N} requires log2(N)={log_N} <= TWO_ADICITY={field.TWO_ADICITY}")
         self.field, self.N, self.log_N = field, N, log_N
         self.omega = root_of_unity(field, N) if omega is None else int(omega)
+        self._tables = None
+
+    def _host_tables(self):
+        if self._tables is None:
+            tw = np.empty(self.N, dtype=self.field.np_dtype)
+            br = np.empty(self.N // 2, dtype=self.field.np_dtype)
+            _check(lib().omr_domain_tables(context(), self.field.id, self.N, self.omega, tw.ctypes.data,
+                                           br.ctypes.data if br.size else None), "omr_domain_tables")
+            self._tables = (tw.astype(np.uint64), br.astype(np.uint64))
+        return self._tables
 
     @property
     def twiddles(self):
-        p, w = self.field.p, self.omega
-        out, cur = [], 1
-        for _ in range(self.N):
-            out.append(cur)
-            cur = cur * w % p
-        return np.array(out, dtype=np.uint64)
+        """twiddles[k] = omega^k for k < N (src/encoder.rs:16)."""
+        return self._host_tables()[0]
+
+    @property
+    def bitrev_twiddles(self):
+        """bitrev_twiddles[i] = omega^bitrev(i, log_N - 1) for i < N/2 (src/encoder.rs:18)."""
+        return self._host_tables()[1]
+
+
+def domain_tables_device(field, n, omega=None):
+    """NttDomain's twiddles and bitrev_twiddles as device tensors (n and n/2
+    words), built on torch's current stream."""
+    import torch
+    omega = root_of_unity(field, n) if omega is None else omega
+    dt = torch.int64 if field.bytes == 8 else torch.int32
+    tw = torch.empty(n, dtype=dt, device="cuda")
+    br = torch.empty(n // 2, dtype=dt, device="cuda")
+    _check(lib().omr_domain_tables_device(context(), field.id, n, omega, _dptr(tw),
+                                          _dptr(br) if n >= 2 else None, _stream_ptr()),
+           "omr_domain_tables_device")
+    return tw, br
 
 
 # ---- device-pointer helpers --------------------------------------------------

@@ -13,6 +13,7 @@
 extern "C" {
 void orc_ntt(int f, uint64_t *buf, uint64_t n, uint64_t omega);
 void orc_dft(int f, uint64_t *buf, uint64_t n, uint64_t omega);
+void orc_domain(int f, uint64_t n, uint64_t omega, uint64_t *twiddles, uint64_t *bitrev_twiddles);
 }
 
 using namespace omrntt;
@@ -116,6 +117,17 @@ void run_field() {
     assert_prefix_agrees_with_ark(DftPartial<F>(s), s, {64, 128, 256});
     assert_prefix_agrees_with_ark(TransformDecomposition<F>(s), s, pw);
     assert_prefix_agrees_with_ark(TransformDecompositionV2<F>(s), s, {1024, 4096, 32768, 65536});
+  }
+  // NttDomain's public tables (src/encoder.rs:14-18) against the oracle's restatement
+  for (size_t n : {(size_t)1, (size_t)2, (size_t)8, (size_t)4096, (size_t)65536}) {
+    NttDomain<F> d(n);
+    std::vector<uint64_t> tw(n), br(n / 2 ? n / 2 : 1);
+    orc_domain(F::id, n, d.omega, tw.data(), br.data());
+    const auto gt = d.twiddles();
+    const auto gb = d.bitrev_twiddles();
+    EXPECT(gt.size() == n && gb.size() == n / 2, "domain table sizes n=%zu", n);
+    for (size_t k = 0; k < gt.size(); k++) EXPECT(gt[k] == tw[k], "twiddles n=%zu k=%zu", n, k);
+    for (size_t i = 0; i < gb.size(); i++) EXPECT(gb[i] == br[i], "bitrev_twiddles n=%zu i=%zu", n, i);
   }
   // registry (src/encoders/mod.rs:32-54)
   EXPECT(encoders::all<F>(12).size() == 6, "all(12)");

@@ -940,3 +940,22 @@ def test_coset_lde_columns_two_streams(fname):
         assert int(nz.item()) > 0, col
         del bad
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("fname", FIELD_IDS)
+def test_domain_tables(fname):
+    """NttDomain's public twiddles / bitrev_twiddles (src/encoder.rs:14-18)
+    built on the device, against the oracle's restatement of NttDomain::new
+    (itself pinned to Python big-int powers), N = 1 .. 2^20, the ark root
+    and a custom root; the device form and the host form agree."""
+    F = field(fname)
+    for logn in (0, 1, 2, 3, 10, 16, 20):
+        n = 1 << logn
+        for omega in (None, pow(E.root_of_unity(F, n), 3, F.p) if n > 2 else None):
+            d = E.NttDomain(n, F, omega)
+            ref = O.Domain(F.id, n, d.omega)
+            assert np.array_equal(d.twiddles, ref.twiddles), (logn, omega)
+            assert np.array_equal(d.bitrev_twiddles, ref.bitrev_twiddles[:n // 2]), (logn, omega)
+            tw, br = E.domain_tables_device(F, n, d.omega)
+            assert np.array_equal(E.from_device(tw, F), ref.twiddles), (logn, omega, "device")
+            assert np.array_equal(E.from_device(br, F), ref.bitrev_twiddles[:n // 2]), (logn, omega, "device")
