@@ -21,6 +21,11 @@ extern "C" {
                          s: u64, word_bytes: u32) -> c_int;
     /// Inverse on a host slice of canonical words of the field's width.
     pub fn omr_intt(ctx: *mut OmrCtx, field: c_int, buf: *mut c_void, n: u64, omega: u64) -> c_int;
+    /// NttDomain's public tables (src/encoder.rs:14-18) as canonical words of
+    /// the field's width: twiddles (n words) and bitrev_twiddles (n/2 words);
+    /// either pointer may be null.
+    pub fn omr_domain_tables(ctx: *mut OmrCtx, field: c_int, n: u64, omega: u64, twiddles: *mut c_void,
+                             bitrev_twiddles: *mut c_void) -> c_int;
     pub fn omr_host_alloc(bytes: usize, out: *mut *mut c_void) -> c_int;
     pub fn omr_host_free(p: *mut c_void) -> c_int;
 }
