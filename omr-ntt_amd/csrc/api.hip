@@ -408,6 +408,34 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     for (int i = 1; i < np; i++) X[i] = ((np - i) % 2 == 1) ? S[0] : S[1];
   }
 
+  // post[i]: pass i applies pass i+1's inter-pass twiddle to its outputs
+  // (kPostTwiddle) and pass i+1 runs without one.  Goldilocks vectors of
+  // 2^22 and more: the later passes are the VALU-bound ones (the first pass
+  // has no twiddle and the most memory stalls), so the chain products move
+  // one pass earlier, where they are cheaper (A/B on one box, forward /
+  // inverse: 2^22 [8,7,7] 67.6 / 66.0 -> 63.8 / 62.0 us, 2^24 [8,8,8]
+  // 195.7 / 195.9 -> 193.3 / 193.4 us); at 2^20 and below the extra table
+  // lookups cost more than that (2^20 30.4 -> 30.8 us, one launch sequence
+  // per vector of the uneven batch 56.7 -> 58.6 ms).  Not on a first pass
+  // with an LDE's zero tail or coset multiplier (the general epilogue), nor
+  // on radix-4096 passes (the four-step kernel, which has no such
+  // epilogue).  An inverse's N^-1 rides in the last such pass's chain.
+  // OMR_POSTTW=0 in the environment turns it off.
+  static const bool post_on = [] {
+    const char *e = getenv("OMR_POSTTW");
+    return !(e && e[0] == '0' && e[1] == 0);
+  }();
+  std::vector<char> post(np, 0);
+  if (post_on && es == 8 && logN >= 22)
+    for (int i = 0; i + 1 < np; i++)
+      post[i] = sch.logR[i] < 12 && (i > 0 || (!o.zero_tail && !o.pre_lo && !o.wide_in));
+  int last_post = -1;
+  for (int i = 0; i + 1 < np; i++)
+    if (post[i]) last_post = i;
+  // the scale rides in the last post pass only when the last pass has no
+  // chain of its own to carry it
+  const bool scale_in_post = o.scale && last_post >= 0 && last_post == np - 2;
+
   int consumed = 0;
   for (int i = 0; i < np; i++) {
     const int logR = sch.logR[i];
@@ -444,12 +472,20 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     a.vec_off_out = (i == np - 1) ? o.off_out : nullptr;
     a.store_limit = (i == np - 1) ? o.store_limit : ~0ull;
     a.flags = 0;
-    if (i > 0) a.flags |= kTwiddle;
+    if (i > 0 && !post[i - 1]) a.flags |= kTwiddle;
+    if (post[i]) {
+      a.flags |= kPostTwiddle;
+      a.post_logR = (uint32_t)sch.logR[i + 1];
+      if (scale_in_post && i == last_post) {
+        a.flags |= kScale;
+        a.scale_w = o.scale_w;
+      }
+    }
     if (i == np - 1) {
       // a last pass with an inter-pass twiddle takes the scale in the start
       // value of its twiddle chains (one product per thread), otherwise
       // every output is scaled
-      if (o.scale) {
+      if (o.scale && !scale_in_post) {
         a.flags |= kScale;
         a.scale_w = o.scale_w;
       }

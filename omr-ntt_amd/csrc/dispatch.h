@@ -54,6 +54,7 @@ inline int pass_epilogue(const PassArgs &a) {
   if (a.flags & kPrefixSum) return kEpiPrefix;
   if (a.pre_lo || a.post_lo || a.nonzero_count || (a.flags & kZeroTail) || a.store_limit < (1ull << a.logN))
     return kEpiGeneral;
+  if (a.flags & kPostTwiddle) return kEpiPost;
   return (a.flags & kScale) ? kEpiScale : kEpiPlain;
 }
 

@@ -45,11 +45,18 @@ struct PassArgs {
   uint32_t log_rmax;          // largest level of the root pyramid
   uint32_t log_mid;
   const void *root_pyr;  // root pyramid: w_(2^L)^m at [2^L + m], m < 2^L, L <= log_rmax (multiplier form)
+  uint32_t post_logR;    // with kPostTwiddle: log radix of the next pass
 };
 // kWideIn / kWideOut (31/32-bit fields): the pass reads / writes its vectors
 // as 64-bit words, one canonical value in the low half of each (a one-limb
 // big-integer layout), narrowing on load and widening on store.
-enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u, kWideIn = 16u, kWideOut = 32u };
+// kPostTwiddle: every output times the inter-pass twiddle that the next
+// pass (radix 2^post_logR, stride s / 2^post_logR) would apply to it; that
+// pass then runs without kTwiddle.  With kScale as well, the outputs are
+// also scaled (the last such pass of an inverse carries its N^-1).
+enum : uint32_t {
+  kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u, kWideIn = 16u, kWideOut = 32u, kPostTwiddle = 64u
+};
 
 // Pass-kernel variants by the options they honour (chosen per launch): the
 // options' per-element tests, if compiled in, cost VALU and split the store
@@ -69,7 +76,8 @@ enum : uint32_t { kTwiddle = 1u, kScale = 2u, kZeroTail = 4u, kPrefixSum = 8u, k
 //                  for m >= 4, so it loads and shifts 4 rows and its first
 //                  register DFT skips the two butterfly levels that only
 //                  see zeros
-enum : int { kEpiPlain = 0, kEpiScale = 1, kEpiGeneral = 2, kEpiPrefix = 3, kEpiLde = 4 };
+//   kEpiPost    -- kTwiddle and kPostTwiddle (a pass before the last)
+enum : int { kEpiPlain = 0, kEpiScale = 1, kEpiGeneral = 2, kEpiPrefix = 3, kEpiLde = 4, kEpiPost = 5 };
 constexpr int kPrefixMaxLogR = 10;  // radices compiled with the prefix epilogue
 
 template <int LOGX>
@@ -539,6 +547,33 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     const bool pruned = EPI == kEpiGeneral && a.store_limit < (1ull << a.logN);
     const T sw = (T)a.scale_w;
     uint32_t nonzero = 0;
+    // kEpiPost: the next pass's inter-pass twiddle w_N^(k2 p2 s2) on output
+    // row krow of column q, where the next pass (radix R2, stride s2 = s/R2)
+    // reads this output as its row k2 = (q / s2) mod R2 of column
+    // p2 = krow L + q / s (L = N / (R s)): the exponent is c0 + c1 krow with
+    // c1 = k2 L s2, c0 = k2 (q / s) s2 (mod N), a power chain over the
+    // thread's rows krow = kk R/EL + t + g RE.
+    T pch[E / EL], pstep = 0;
+    if constexpr (EPI == kEpiPost) {
+      const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
+      const uint32_t lomask = (1u << a.lo_bits) - 1;
+      const uint64_t nmask = (1ull << a.logN) - 1;
+      auto twN = [&](uint64_t e) -> T {  // w_N^e
+        e &= nmask;
+        return F::wmul(tlo[(uint32_t)e & lomask], thi[e >> a.lo_bits]);
+      };
+      const uint32_t logS2 = logS - a.post_logR;
+      const uint32_t logL = a.logN - LOGR - logS;
+      const uint64_t k2 = (q >> logS2) & ((1u << a.post_logR) - 1);
+      const uint64_t c1 = k2 << (logL + logS2);
+      const uint64_t c0 = (k2 * (uint64_t)(q >> logS)) << logS2;
+      pstep = twN(c1 * (uint64_t)(R / EL));
+#pragma unroll
+      for (int g = 0; g < E / EL; g++) {
+        pch[g] = twN(c0 + c1 * (uint64_t)(t + g * RE));
+        if (a.flags & kScale) pch[g] = F::wmul(pch[g], (T)a.scale_w);  // an inverse's N^-1, once per element
+      }
+    }
     // WIDE: 64-bit output words (a uniform choice: the loop is compiled twice)
     auto store_all = [&](auto wide_tag) {
       constexpr bool WIDE = decltype(wide_tag)::value;
@@ -548,6 +583,10 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
         for (int kk = 0; kk < EL; kk++) {
           const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
           T v = x[g * EL + kk];
+          if constexpr (EPI == kEpiPost) {
+            v = F::mulw(v, pch[g]);
+            if (kk + 1 < EL) pch[g] = F::wmul(pch[g], pstep);
+          }
           if (scale) v = F::mulw(v, sw);
           const uint32_t oidx = (krow << logNR) + q;
           if (EPI == kEpiGeneral && a.post_lo) {

@@ -56,8 +56,10 @@ def train(X, times, seed=0, steps=3000, lr=3e-3):
     return m, loss
 
 
-def regrets(model, shapes, times):
-    """Per-shape regret of the model's choice and of the default class."""
+def regrets(model, shapes, times, top_k=3):
+    """Per-shape regret of the model's choice, of the fastest of its top_k
+    predicted classes (what autotune_shape(top_k=...) installs after timing
+    those k), and of the default class."""
     out = []
     for (F, n, out_len, b), t in zip(shapes, times):
         best = t[t > 0].min()
@@ -65,9 +67,14 @@ def regrets(model, shapes, times):
         c_def = default_class(F, n, out_len)
         r_model = t[c_model] / best - 1.0
         r_def = (t[c_def] / best - 1.0) if c_def >= 0 and t[c_def] > 0 else float("nan")
+        pred = model.forward(features(F, n, out_len, b))[0]
+        valid = np.flatnonzero(t > 0)
+        topk = valid[np.argsort(pred[valid])[:top_k]]
+        r_topk = t[topk].min() / best - 1.0
         out.append({"field": F.name, "n": n, "out_len": out_len, "batch": b, "model_class": c_model,
                     "default_class": c_def, "best_class": int(np.argmax(np.where(t > 0, -t, -np.inf))),
-                    "regret_model": float(r_model), "regret_default": float(r_def)})
+                    "regret_model": float(r_model), "regret_default": float(r_def),
+                    f"regret_top{top_k}": float(r_topk)})
     return out
 
 
@@ -100,8 +107,10 @@ def experiment(shapes=None, reps=3, seed=0, steps=3000):
 def _summary(held):
     rm = np.array([r["regret_model"] for r in held])
     rd = np.array([r["regret_default"] for r in held])
+    r3 = np.array([r["regret_top3"] for r in held])
     return {"shapes": len(held), "mean_regret_model": float(np.mean(rm)),
             "mean_regret_default": float(np.nanmean(rd)),
+            "mean_regret_top3": float(np.mean(r3)),
             "max_regret_model": float(np.max(rm)),
             "frac_within_10pct_model": float(np.mean(rm <= 0.10)),
             "frac_within_10pct_default": float(np.nanmean(rd <= 0.10))}
