@@ -420,7 +420,8 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
   // with an LDE's zero tail or coset multiplier (the general epilogue), nor
   // on radix-4096 passes (the four-step kernel, which has no such
   // epilogue).  An inverse's N^-1 rides in the last such pass's chain.
-  // OMR_POSTTW=0 in the environment turns it off.
+  // (Moving only the first pass's twiddle into pass 0: 2^22 66.0 us, between
+  // the two.)  OMR_POSTTW=0 in the environment turns it off.
   static const bool post_on = [] {
     const char *e = getenv("OMR_POSTTW");
     return !(e && e[0] == '0' && e[1] == 0);
