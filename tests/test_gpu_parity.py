@@ -959,3 +959,42 @@ def test_domain_tables(fname):
             tw, br = E.domain_tables_device(F, n, d.omega)
             assert np.array_equal(E.from_device(tw, F), ref.twiddles), (logn, omega, "device")
             assert np.array_equal(E.from_device(br, F), ref.bitrev_twiddles[:n // 2]), (logn, omega, "device")
+
+
+def test_post_twiddle_batches_and_odd_sizes():
+    """Goldilocks vectors of 2^22 and more take each next pass's twiddle one
+    pass early (the first passes' outputs carry it; an inverse's N^-1 rides
+    in the last such chain): batches of 2^22, 2^23 (odd log N, a radix-128
+    last pass) and explicit schedules, forward and inverse, in place and
+    out of place, against the oracle."""
+    F = E.Goldilocks
+    for logn, batch in ((22, 3), (23, 1)):
+        n = 1 << logn
+        print(f"case: post-twiddle Goldilocks 2^{logn} x {batch}", flush=True)
+        x = inputs(F, n * batch, 3300 + logn)
+        t = E.to_device(x, F)
+        out = torch.empty_like(t)
+        E.ntt_device(t, out, F, batch=batch, n=n)
+        got = E.from_device(out, F)
+        for b in range(batch):
+            assert np.array_equal(got[b * n:(b + 1) * n], O.ntt(F.id, x[b * n:(b + 1) * n])), (logn, b)
+        back = torch.empty_like(t)
+        E.intt_device(out, back, F, batch=batch, n=n)
+        assert torch.equal(back, t), (logn, "inverse")
+        buf = t.clone()
+        E.ntt_device(buf, buf, F, batch=batch, n=n)
+        assert torch.equal(buf, out), (logn, "in place")
+        E.intt_device(buf, buf, F, batch=batch, n=n)
+        assert torch.equal(buf, t), (logn, "inverse in place")
+        del out, back, buf
+    n = 1 << 22
+    x = inputs(F, n, 3399)
+    X = O.ntt(F.id, x)
+    t = E.to_device(x, F)
+    for sched in ([6, 8, 8], [11, 11], [4, 6, 6, 6], [12, 10], [10, 12]):
+        o2 = torch.empty_like(t)
+        E.ntt_schedule_device(t, o2, F, sched)
+        assert np.array_equal(E.from_device(o2, F), X), sched
+        E.ntt_schedule_device(o2, o2, F, sched, inverse=True)
+        assert torch.equal(o2, t), (sched, "inverse")
+    torch.cuda.empty_cache()
