@@ -229,12 +229,24 @@ struct PassKernel {
 #endif
 __device__ __forceinline__ uint32_t lpad(uint32_t i) { return OMR_LDS_PAD ? i + (i >> 5) : i; }
 constexpr int lds_padded(int n) { return OMR_LDS_PAD ? n + n / 32 + 1 : n; }
+// lpad((row << logC) + col) for col < 2^logC.  FOLD (logC known at compile
+// time): split so that the row term is linear, and a row step of a
+// compile-time amount becomes an instruction offset instead of a
+// per-element shift.  With a run-time logC the split form only costs more.
+template <bool FOLD>
+__device__ __forceinline__ uint32_t lrc(uint32_t row, uint32_t col, uint32_t logC) {
+  if (!OMR_LDS_PAD) return (row << logC) + col;
+  if constexpr (!FOLD) return lpad((row << logC) + col);
+  if (logC > 5) return (row << logC) + (row << (logC - 5)) + col + (col >> 5);
+  if (logC == 5) return (row << 5) + row + col;  // col < 32
+  return (row << logC) + (row >> (5 - logC)) + col;
+}
 
 // In-tile stage ST >= 1 of the R-point Stockham: reads the previous stage's
 // outputs from LDS, applies w_R^(k' p s), runs the register DFTs.  The
 // previous stage's registers are written by thread (tw, cw) -- its own
 // (t, c), or for stage 1 of a row-fast pass the load mapping's.
-template <class F, int LOGR, int LOGE, int ST, int U = 0>
+template <class F, int LOGR, int LOGE, int ST, int U = 0, bool FOLD = false>
 __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
                                            const typename F::T (&wE)[(1 << LOGE) / 2 + 1],
                                            typename F::T *lds, const typename F::T *root,
@@ -249,7 +261,7 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
     constexpr int LOGS = LOGR - ST * LOGE - LOGEI;  // in-tile stride of this stage
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < K::E; m++) lds[lpad(((tw + m * K::RE) << logC) + cw)] = x[m];
+    for (int m = 0; m < K::E; m++) lds[lrc<FOLD>(tw + m * K::RE, cw, logC)] = x[m];
     __syncthreads();
 #pragma unroll
     for (int g = 0; g < K::E / EI; g++) {
@@ -259,14 +271,14 @@ __device__ __forceinline__ void tile_stage(typename F::T (&x)[1 << LOGE],
 #pragma unroll
       for (int kk = 0; kk < EI; kk++) {
         const uint32_t row = ps * EI + ((uint32_t)kk << LOGS) + jj;
-        T v = lds[lpad((row << logC) + c)];
+        T v = lds[lrc<FOLD>(row, c, logC)];
         const uint32_t e = (uint32_t)kk * ps;
         if (kk != 0) v = F::mulw(v, root[e << root_shift]);  // e = 0: times root[0] = 1
         x[g * EI + kk] = v;
       }
     }
     GroupDft<F, LOGE, LOGEI, K::E / EI, U>::run(x, wE);
-    tile_stage<F, LOGR, LOGE, ST + 1, U>(x, wE, lds, root, t, c, logC, root_shift, t, c);
+    tile_stage<F, LOGR, LOGE, ST + 1, U, FOLD>(x, wE, lds, root, t, c, logC, root_shift, t, c);
   }
 }
 
@@ -284,11 +296,44 @@ __device__ __forceinline__ T ld_stream(const T *p) {
   return __builtin_nontemporal_load(p);
 }
 
+// Raw buffer access to one vector of less than 2^32 bytes (the descriptor's
+// record count is 32-bit and an access past it reads 0 and drops the store,
+// so a 2^32-byte vector keeps 64-bit addresses): a descriptor over
+// the vector, a 32-bit per-thread byte offset, and the uniform part of each
+// row's offset in an SGPR (the hardware adds the two).  A thread's E rows
+// then cost no per-element 64-bit address arithmetic and their offsets one
+// VGPR between them instead of two per row.  (gfx9 descriptor word 3:
+// 32-bit data format, no swizzle; raw buffers are byte-addressed.)
+typedef unsigned int omr_u32x2 __attribute__((__vector_size__(8)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vec_rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+// NT: the nontemporal hint of ld_stream (cache-policy bit 1 on gfx9)
+template <class T, bool NT>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  constexpr int aux = NT ? 2 : 0;
+  if constexpr (sizeof(T) == 4) {
+    return (T)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, aux);
+  } else {
+    const omr_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, aux);
+    return ((T)v[1] << 32) | (T)v[0];
+  }
+}
+template <class T>
+__device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)voff, (int)soff, 0);
+  } else {
+    const omr_u32x2 w = {(uint32_t)v, (uint32_t)((uint64_t)v >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)voff, (int)soff, 0);
+  }
+}
+
 template <class F, int LOGR, int LOGE, int EPI>
 __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const typename F::T *__restrict__ src,
                                           const uint64_t *__restrict__ srcw, bool wide, const PassArgs &a,
                                           uint32_t tile, uint32_t tid, uint32_t c, uint32_t t, bool staged,
-                                          uint32_t logC) {
+                                          uint32_t logC, bool buf, __amdgpu_buffer_rsrc_t rs) {
   using K = PassKernel<F, LOGR, LOGE>;
   using T = typename F::T;
   const uint32_t logS = a.logS;
@@ -306,7 +351,8 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
     for (int m = 0; m < K::E; m++) {
       if (m < K::E / 4) {
         const uint32_t pos = pos0 + (uint32_t)m * step;
-        if constexpr (sizeof(T) == 4) xn[m] = wide ? (T)ld_stream(srcw + pos) : ld_stream(src + pos);
+        if (buf && !wide) xn[m] = buf_load<T, true>(rs, pos0 * (uint32_t)sizeof(T), (uint32_t)m * step * (uint32_t)sizeof(T));
+        else if constexpr (sizeof(T) == 4) xn[m] = wide ? (T)ld_stream(srcw + pos) : ld_stream(src + pos);
         else xn[m] = ld_stream(src + pos);
       } else {
         xn[m] = 0;
@@ -337,6 +383,10 @@ __device__ __forceinline__ void load_tile(typename F::T (&xn)[1 << LOGE], const 
       const uint32_t pos = pos0 + (uint32_t)m * step;
       xn[m] = ((uint64_t)pos < valid) ? ld_stream(src + pos) : (T)0;
     }
+  } else if (buf) {
+#pragma unroll
+    for (int m = 0; m < K::E; m++)
+      xn[m] = buf_load<T, true>(rs, pos0 * (uint32_t)sizeof(T), (uint32_t)m * step * (uint32_t)sizeof(T));
   } else {
 #pragma unroll
     for (int m = 0; m < K::E; m++) xn[m] = ld_stream(src + pos0 + (uint32_t)m * step);
@@ -358,6 +408,9 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   __shared__ T lds[lds_padded(K::LDS_N)];
 
   const uint32_t logC = FIXC ? (uint32_t)__builtin_ctz(K::CMAX) : a.logC;
+  // 4-byte fields in full narrow tiles (C >= 16 columns, compile-time logC):
+  // LDS indices folded into instruction offsets, buffer loads and stores
+  constexpr bool kNarrow4 = sizeof(T) == 4 && FIXC && !K::WIDE;
   const uint32_t C = 1u << logC;
   const uint32_t tid = threadIdx.x;
   const uint32_t c = tid & (C - 1), t = tid >> logC;
@@ -398,6 +451,14 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     const bool wide_in = sizeof(T) == 4 && (a.flags & kWideIn) != 0;
     const uint64_t *__restrict__ srcw = (const uint64_t *)a.src + voff_in;
     uint64_t *__restrict__ dstw = (uint64_t *)a.dst + voff_out;
+    // Buffer loads and stores: vectors of less than 2^32 bytes in the
+    // kNarrow4 kernels.  Measured on MI355X against 64-bit addresses:
+    // BabyBear / KoalaBear 2^24 -6 %, the 256 x 2^20 blowup-4 encoder -10 %
+    // (VGPRs 66 -> 38 at radix 256); Goldilocks 2^24 +2 %, the wide
+    // radix-2^11 tiles +4 %.
+    const bool buf = kNarrow4 && a.logN + 2u <= 31u;
+    const uint32_t vbytes = buf ? (uint32_t)sizeof(T) << a.logN : 0u;
+    const __amdgpu_buffer_rsrc_t rs_in = vec_rsrc(src, vbytes), rs_out = vec_rsrc(dst, vbytes);
     if constexpr (EPI == kEpiPrefix) {  // tiles whose p has no k' = p + P k < out_len
       if (((tile << logC) >> logS) >= (uint32_t)a.store_limit) return;
     }
@@ -405,7 +466,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     // raised issue priority while the tile's loads go out: a wave starting
     // a tile gets its memory requests in ahead of waves in their arithmetic
     __builtin_amdgcn_s_setprio(1);
-    load_tile<F, LOGR, LOGE, EPI>(x, src, srcw, wide_in, a, tile, tid, c0, t0, stage_lds, logC);
+    load_tile<F, LOGR, LOGE, EPI>(x, src, srcw, wide_in, a, tile, tid, c0, t0, stage_lds, logC, buf, rs_in);
     __builtin_amdgcn_s_setprio(0);
 
     // column, p and first input position of the load mapping
@@ -416,17 +477,26 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     if (stage_lds) {
       // element i = (pp R + k) S + jj of the block is column pp S + jj, row k
       __syncthreads();
+      if (kNarrow4 && logS + LOGE <= logC) {
+        // i = tid + m nthr with nthr / S a multiple of R: the thread's row k
+        // is the same for every m and its column steps by C / E
+        const uint32_t k = (tid >> logS) & (R - 1);
+        const uint32_t col = ((tid >> (logS + LOGR)) << logS) + (tid & (S - 1));
 #pragma unroll
-      for (int m = 0; m < E; m++) {
-        const uint32_t i = tid + (uint32_t)m * nthr;
-        const uint32_t jj = i & (S - 1);
-        const uint32_t k = (i >> logS) & (R - 1);
-        const uint32_t pp = i >> (logS + LOGR);
-        lds[lpad((k << logC) + (pp << logS) + jj)] = x[m];
+        for (int m = 0; m < E; m++) lds[lrc<kNarrow4>(k, col + ((uint32_t)m << (logC >= (uint32_t)LOGE ? logC - LOGE : 0)), logC)] = x[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+          const uint32_t i = tid + (uint32_t)m * nthr;
+          const uint32_t jj = i & (S - 1);
+          const uint32_t k = (i >> logS) & (R - 1);
+          const uint32_t pp = i >> (logS + LOGR);
+          lds[lrc<kNarrow4>(k, (pp << logS) + jj, logC)] = x[m];
+        }
       }
       __syncthreads();
 #pragma unroll
-      for (int m = 0; m < E; m++) x[m] = lds[lpad(((t + m * RE) << logC) + c)];
+      for (int m = 0; m < E; m++) x[m] = lds[lrc<kNarrow4>(t + m * RE, c, logC)];
     }
 
     if ((EPI == kEpiGeneral || EPI == kEpiLde) && a.pre_lo) {  // h^pos on input position pos (coset shift)
@@ -483,7 +553,7 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
     // in-tile stage 0: stride R/E, one group per thread, no in-tile twiddle
     if constexpr (EPI == kEpiLde && K::NSTAGE >= 2) dft_reg_nz4<F, LOGE, U>(x, wE);
     else dft_reg<F, LOGE, (K::NSTAGE == 1 ? K::LOGLAST : LOGE), 0, U>(x, wE);
-    tile_stage<F, LOGR, LOGE, 1, U>(x, wE, lds, rootR, t, c, logC, 0, t0, c0);
+    tile_stage<F, LOGR, LOGE, 1, U, kNarrow4>(x, wE, lds, rootR, t, c, logC, 0, t0, c0);
     const uint32_t q = (tile << logC) + c;  // the store mapping's column
 
     // register slot (g, kk) holds tile output row kk (R/EL) + t + g (R/E)
@@ -595,6 +665,9 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
           }
           if (!pruned || (uint64_t)oidx < a.store_limit) {
             if constexpr (WIDE) dstw[oidx] = (uint64_t)v;
+            else if (buf)  // oidx = (t << logNR) + q + (the row's uniform part << logNR)
+              buf_store<T>(v, rs_out, ((t << logNR) + q) * (uint32_t)sizeof(T),
+                           (((uint32_t)kk * (R / EL) + (uint32_t)g * RE) << logNR) * (uint32_t)sizeof(T));
             else dst[oidx] = v;
           } else {
             nonzero += (v != 0);
