@@ -95,7 +95,7 @@ __device__ __forceinline__ constexpr int brev_c(int i) {
 template <class F, int U, int LOGE>
 __device__ __forceinline__ typename F::T diff_mul(typename F::T u, typename F::T v, typename F::T c, int idx) {
   if constexpr (U != 0) return F::diff_pow2(u, v, (uint32_t)((U * idx * (64 >> LOGE)) % 192));
-  else return F::mulw(F::sub(u, v), c);
+  else return F::diff_mulw(u, v, c);
 }
 
 // Register-DFT constants w_E^k, k <= E/2: multiplier-form values from the

@@ -68,7 +68,9 @@ struct Mont32 {
   }
   OMR_HD static T sub(T a, T b) { return fix_diff(a - b, a < b); }
   OMR_HD static T neg(T a) { return a ? P - a : 0; }
-  // Montgomery product a * b * 2^-32 mod p for a < p, b < 2^32.
+  // Montgomery product a * b * 2^-32 mod p for a < p, b < 2^32 (or, for
+  // p < 2^31, any a < 2^32 with b < p: either way a b < p 2^32, so the
+  // high-half difference lies in (-p, p)).
   OMR_HD static T mont(T a, T b) {
     uint64_t t = (uint64_t)a * b;
     uint32_t tlo = (uint32_t)t, thi = (uint32_t)(t >> 32);
@@ -77,6 +79,14 @@ struct Mont32 {
     return fix_diff(thi - mh, thi < mh);
   }
   OMR_HD static T mulw(T a, T w) { return mont(a, w); }
+  // (a - b) * w for canonical a, b and multiplier w < p.  For p < 2^31 the
+  // unreduced a + (p - b) < 2p < 2^32 is a valid Montgomery operand: with
+  // w < p the product stays below p 2^32, so mont's result is canonical.
+  // One instruction fewer than mulw(sub(a, b), w).
+  OMR_HD static T diff_mulw(T a, T b, T w) {
+    if constexpr (!BIG) return mont(a + (P - b), w);
+    else return mont(sub(a, b), w);
+  }
   OMR_HD static T wmul(T w1, T w2) { return mont(w1, w2); }  // W x W -> W
   OMR_HD static T to_w(T a) { return mont(a, R2); }
   OMR_HD static T from_w(T w) { return mont(w, 1); }
@@ -147,6 +157,7 @@ struct Goldilocks {
     return reduce128(lo, hi);
   }
   OMR_HD static T mulw(T a, T w) { return mul(a, w); }
+  OMR_HD static T diff_mulw(T a, T b, T w) { return mul(sub(a, b), w); }
   OMR_HD static T wmul(T w1, T w2) { return mul(w1, w2); }
   OMR_HD static T to_w(T a) { return a; }
   OMR_HD static T from_w(T w) { return w; }

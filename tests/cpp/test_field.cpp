@@ -1,6 +1,7 @@
 // Host build of the engine's field arithmetic (omr-ntt_amd/csrc/field.h)
 // checked against plain 128-bit remainders: add/sub, Montgomery and
-// Goldilocks products, multiplier-form round trips, Goldilocks x * 2^k,
+// Goldilocks products, multiplier-form round trips, the butterfly
+// difference times a multiplier, Goldilocks x * 2^k,
 // (a - b) * 2^k with the sign folded, x * 2^-m.
 #include <cstdio>
 #include <random>
@@ -22,7 +23,7 @@ void check_field(std::mt19937_64 &rng) {
   const uint64_t p = F::P;
   uint64_t edges[] = {0, 1, 2, p - 1, p - 2, p / 2, p / 2 + 1, 0xFFFFFFFFull % p, (1ull << 31) % p};
   for (int it = 0; it < 200000; it++) {
-    uint64_t a = it < 81 ? edges[it / 9] : rng() % p, b = it < 81 ? edges[it % 9] : rng() % p;
+    uint64_t a = it < 729 ? edges[it % 81 / 9] : rng() % p, b = it < 729 ? edges[it % 9] : rng() % p;
     using T = typename F::T;
     T x = (T)a, y = (T)b;
     CHECK((uint64_t)F::add(x, y) == (uint64_t)(((u128)a + b) % p), "add %d\n", F::ID);
@@ -32,6 +33,12 @@ void check_field(std::mt19937_64 &rng) {
     CHECK((uint64_t)F::from_w(F::to_w(x)) == a, "w round trip %d\n", F::ID);
     CHECK((uint64_t)F::mulw(x, F::to_w(y)) == (uint64_t)(((u128)a * b) % p), "mulw %d\n", F::ID);
     CHECK((uint64_t)F::from_w(F::wmul(F::to_w(x), F::to_w(y))) == (uint64_t)(((u128)a * b) % p), "wmul %d\n", F::ID);
+    // (a - b) w for a multiplier w of c: the unreduced difference a + p - b
+    // reaches 2p - 1 (a = p - 1, b = 0), and c runs over the edges too
+    const uint64_t c = it < 729 ? edges[(it / 81) % 9] : rng() % p;
+    CHECK((uint64_t)F::diff_mulw(x, y, F::to_w((T)c)) == (uint64_t)((((u128)a + p - b) % p * c) % p),
+          "diff_mulw %d %llu %llu %llu\n", F::ID, (unsigned long long)a, (unsigned long long)b,
+          (unsigned long long)c);
   }
 }
 
