@@ -427,9 +427,16 @@ omr_status run_transform(omr_ctx *ctx, int f, const void *d_in, void *d_out, uin
     return !(e && e[0] == '0' && e[1] == 0);
   }();
   std::vector<char> post(np, 0);
-  if (post_on && es == 8 && logN >= 22)
-    for (int i = 0; i + 1 < np; i++)
-      post[i] = sch.logR[i] < 12 && (i > 0 || (!o.zero_tail && !o.pre_lo && !o.wide_in));
+  if (post_on && logN >= 22)
+    for (int i = 0; i + 1 < np; i++) {
+      bool ok = sch.logR[i] < 12 && (i > 0 || (!o.zero_tail && !o.pre_lo && !o.wide_in));
+      if (es == 4) {
+        int le = sch.logE;
+        if (!has_pass_kernel(sch.logR[i], le, es)) le = auto_log_e(sch.logR[i]);
+        ok = ok && post4_ok(sch.logR[i], le, pass_log_cols((int)logN, sch.logR[i], le, es));
+      }
+      post[i] = ok;
+    }
   int last_post = -1;
   for (int i = 0; i + 1 < np; i++)
     if (post[i]) last_post = i;

@@ -58,6 +58,13 @@ inline int pass_epilogue(const PassArgs &a) {
   return (a.flags & kScale) ? kEpiScale : kEpiPlain;
 }
 
+// Whether a 4-byte-field pass can carry the next pass's twiddle (kEpiPost
+// is compiled for those fields at register radix 16, pass radix 2^6..2^11,
+// full tiles only).
+inline bool post4_ok(int logR, int logE, int logC) {
+  return logE == 4 && logR >= 6 && logR <= 11 && (1 << logC) == pass_cmax(logR, 4, logE);
+}
+
 // Whether a kEpiGeneral pass is the first pass of a blowup-4 coset
 // extension that kEpiLde runs: zero tail from N/4 on, an input multiplier,
 // nothing else; register radix 16 with at least two in-tile stages; tiles

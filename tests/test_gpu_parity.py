@@ -961,16 +961,19 @@ def test_domain_tables(fname):
             assert np.array_equal(E.from_device(br, F), ref.bitrev_twiddles[:n // 2]), (logn, omega, "device")
 
 
-def test_post_twiddle_batches_and_odd_sizes():
-    """Goldilocks vectors of 2^22 and more take each next pass's twiddle one
-    pass early (the first passes' outputs carry it; an inverse's N^-1 rides
-    in the last such chain): batches of 2^22, 2^23 (odd log N, a radix-128
-    last pass) and explicit schedules, forward and inverse, in place and
-    out of place, against the oracle."""
-    F = E.Goldilocks
+@pytest.mark.parametrize("fname", ["Goldilocks", "BabyBear", "TeddyBear"])
+def test_post_twiddle_batches_and_odd_sizes(fname):
+    """Vectors of 2^22 and more take each next pass's twiddle one pass early
+    (the first passes' outputs carry it; an inverse's N^-1 rides in the last
+    such chain; 31/32-bit fields at register radix 16 and pass radix
+    2^6..2^11 only, other passes keep their own twiddle): batches of 2^22,
+    2^23 (odd log N) and explicit schedules mixing passes that can and
+    cannot carry it, forward and inverse, in place and out of place, against
+    the oracle."""
+    F = field(fname)
     for logn, batch in ((22, 3), (23, 1)):
         n = 1 << logn
-        print(f"case: post-twiddle Goldilocks 2^{logn} x {batch}", flush=True)
+        print(f"case: post-twiddle {fname} 2^{logn} x {batch}", flush=True)
         x = inputs(F, n * batch, 3300 + logn)
         t = E.to_device(x, F)
         out = torch.empty_like(t)
@@ -991,7 +994,11 @@ def test_post_twiddle_batches_and_odd_sizes():
     x = inputs(F, n, 3399)
     X = O.ntt(F.id, x)
     t = E.to_device(x, F)
-    for sched in ([6, 8, 8], [11, 11], [4, 6, 6, 6], [12, 10], [10, 12]):
+    scheds = [[6, 8, 8], [11, 11], [4, 6, 6, 6], [12, 10], [10, 12]]
+    if F.bytes == 4:
+        scheds += [[5, 9, 8], [8, 8, 6], [13, 9], [9, 13], [7, 7, 8]]
+    for sched in scheds:
+        print(f"case: post-twiddle {fname} 2^22 schedule {sched}", flush=True)
         o2 = torch.empty_like(t)
         E.ntt_schedule_device(t, o2, F, sched)
         assert np.array_equal(E.from_device(o2, F), X), sched
