@@ -374,9 +374,22 @@ def _dispatch_names(run_dir, match):
                 if match(name)}
 
 
-def summarize_profiles(prof_dir, out_path, logn=24):
-    """Per-pass HBM traffic and VALU work of the 2^N Goldilocks forward
-    transform from five rocprofv3 --pmc runs under prof_dir (copy_fetch,
+# pass-kernel name prefixes of the forward transform per field (Goldilocks:
+# the forward instantiations, register-DFT unit 39; the 31/32-bit fields use
+# one set of kernels for both directions)
+FWD_KERNEL = {"Goldilocks": "k_stockham_pass<omr::Goldilocks,", "BabyBear": "k_stockham_pass<omr::Mont32<2013265921u",
+              "KoalaBear": "k_stockham_pass<omr::Mont32<2130706433u", "TeddyBear": "k_stockham_pass<omr::Mont32<3221225473u"}
+
+
+def is_fwd_pass(name, field="Goldilocks"):
+    if FWD_KERNEL[field] not in name:
+        return False
+    return field != "Goldilocks" or ", 39," in name or ", 39>" in name
+
+
+def summarize_profiles(prof_dir, out_path, logn=24, field="Goldilocks"):
+    """Per-pass HBM traffic and VALU work of the 2^N forward transform of
+    `field` (Goldilocks by default) from five rocprofv3 --pmc runs under prof_dir (copy_fetch,
     copy_write: `bench.py --copy-calibration`; fetch, write, valu: the bench
     itself), scaled by the copy of known size.  Writes the JSON bench.py
     reads for roofline.traffic and valu, and returns it."""
@@ -385,8 +398,8 @@ def summarize_profiles(prof_dir, out_path, logn=24):
     def is_copy(name):
         return "copy" in name.lower()
 
-    def fwd(name):  # the forward transform's pass kernels: Goldilocks, register-DFT unit 39
-        return "k_stockham_pass<omr::Goldilocks," in name and (", 39," in name or ", 39>" in name)
+    def fwd(name):  # the forward transform's pass kernels
+        return is_fwd_pass(name, field)
     cf = _pmc_per_dispatch(os.path.join(prof_dir, "copy_fetch"), "FETCH_SIZE", is_copy)
     cw = _pmc_per_dispatch(os.path.join(prof_dir, "copy_write"), "WRITE_SIZE", is_copy)
     fetch_scale = copy_bytes / 1024 / (sum(cf.values()) / len(cf))
@@ -396,7 +409,8 @@ def summarize_profiles(prof_dir, out_path, logn=24):
     v = _pmc_per_dispatch(os.path.join(prof_dir, "valu"), "SQ_INSTS_VALU", fwd)
     sys.path.insert(0, os.path.join(ROOT, "omr-ntt_amd"))
     import omr_ntt as E
-    sched = E.default_schedule(E.Goldilocks, n)
+    F = getattr(E, field)
+    sched = E.default_schedule(F, n)
     npass = len(sched)
 
     def per_pass(d):  # skip the first transform (warm-up), average each pass index
@@ -407,7 +421,7 @@ def summarize_profiles(prof_dir, out_path, logn=24):
     passes = [{"pass": i, "fetch_KiB_raw": fk[i], "write_KiB_raw": wk[i],
                "fetch_bytes": fk[i] * 1024 * fetch_scale, "write_bytes": wk[i] * 1024 * write_scale,
                "valu_wave_instr": vi[i], "valu_lane_ops_per_elem": vi[i] * 64 / n} for i in range(npass)]
-    res = {"kernel": "k_stockham_pass", "workload": f"Goldilocks forward NTT 2^{logn}",
+    res = {"kernel": "k_stockham_pass", "workload": f"{field} forward NTT 2^{logn}",
            "schedule_log_radix": sched,
            "traffic_bytes_per_launch": sum(p["fetch_bytes"] + p["write_bytes"] for p in passes) / npass,
            "fetch_scale": fetch_scale, "write_scale": write_scale, "per_pass": passes}
@@ -492,7 +506,7 @@ def summarize_encoder(prof_dir):
           f"scale factors FETCH_SIZE x{fetch_scale:.3f}, WRITE_SIZE x{write_scale:.3f}")
 
 
-def summarize_sq(sq_dir, clk_dir, logn=24):
+def summarize_sq(sq_dir, clk_dir, logn=24, field="Goldilocks"):
     """Per-pass SQ counters of the forward transform (Goldilocks, U = 39
     instantiation) from two rocprofv3 --pmc runs of this bench: the wait / LDS
     counters, and GRBM_COUNT with kernel durations (--kernel-trace) for the
@@ -508,14 +522,14 @@ def summarize_sq(sq_dir, clk_dir, logn=24):
             dur = dict(c.execute("select dispatch_id, duration from kernels").fetchall()) if with_dur else {}
         per = {}
         for did, name, cn, v in pm:
-            if "k_stockham_pass<omr::Goldilocks," in name and (", 39," in name or ", 39>" in name):
+            if is_fwd_pass(name, field):
                 per.setdefault(did, {})
                 per[did][cn] = per[did].get(cn, 0.0) + float(v)
         return per, dur
 
     sys.path.insert(0, os.path.join(ROOT, "omr-ntt_amd"))
     import omr_ntt as E
-    npass = len(E.default_schedule(E.Goldilocks, 1 << logn))
+    npass = len(E.default_schedule(getattr(E, field), 1 << logn))
     sq, _ = load(sq_dir)
     ck, dur = load(clk_dir, True)
 
@@ -540,7 +554,7 @@ def summarize_sq(sq_dir, clk_dir, logn=24):
               f"{sqp[p]['SQ_LDS_BANK_CONFLICT'] / max(sqp[p]['SQ_ACTIVE_INST_LDS'], 1):.2f} |")
 
 
-def summarize_trace(trace_dir, logn=24):
+def summarize_trace(trace_dir, logn=24, field="Goldilocks"):
     """Kernel-duration table of a rocprofv3 --kernel-trace run of this bench
     (calls, mean/min/max microseconds per kernel) and the forward
     transform's mean per pass in launch order, as markdown."""
@@ -565,9 +579,8 @@ def summarize_trace(trace_dir, logn=24):
         print(f"| `{name[:90]}` | {k} | {tot / k:.1f} | {lo:.1f} | {hi:.1f} | {vg} | {lds} |")
     sys.path.insert(0, os.path.join(ROOT, "omr-ntt_amd"))
     import omr_ntt as E
-    npass = len(E.default_schedule(E.Goldilocks, 1 << logn))
-    fwd = [d / 1e3 for name, d, _, _, _ in rows
-           if "k_stockham_pass<omr::Goldilocks," in name and (", 39," in name or ", 39>" in name)]
+    npass = len(E.default_schedule(getattr(E, field), 1 << logn))
+    fwd = [d / 1e3 for name, d, _, _, _ in rows if is_fwd_pass(name, field)]
     fwd = fwd[npass:len(fwd) // npass * npass]  # skip the first transform
     if fwd:
         per = [sum(fwd[i::npass]) / len(fwd[i::npass]) for i in range(npass)]
@@ -581,6 +594,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--logn", type=int, default=24)
+    ap.add_argument("--field", default="Goldilocks", choices=sorted(FWD_KERNEL),
+                    help="field of the headline transform (for counter profiles of the other fields' pass "
+                         "kernels; the headline metric is Goldilocks)")
     ap.add_argument("--schedule", default=None,
                     help="comma-separated log2 pass radices for the headline transform (e.g. 12,12 for the "
                          "four-step 2^12 x 2^12), to profile a schedule; default: the engine's tuned one")
@@ -616,13 +632,13 @@ def main():
         print(f"bench.py: unknown --extras legs {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
     if args.summarize_profiles:
-        summarize_profiles(args.summarize_profiles, args.profile_out, args.logn)
+        summarize_profiles(args.summarize_profiles, args.profile_out, args.logn, args.field)
         return
     if args.summarize_sq:
-        summarize_sq(*args.summarize_sq, args.logn)
+        summarize_sq(*args.summarize_sq, args.logn, args.field)
         return
     if args.summarize_trace:
-        summarize_trace(args.summarize_trace, args.logn)
+        summarize_trace(args.summarize_trace, args.logn, args.field)
         return
     if args.summarize_encoder:
         summarize_encoder(args.summarize_encoder)
@@ -651,11 +667,11 @@ def main():
         encoder_only(E, torch, ENC_REPS)
         return
 
-    F = E.Goldilocks
+    F = getattr(E, args.field)
     n = 1 << args.logn
     sched = [int(v) for v in args.schedule.split(",")] if args.schedule else E.default_schedule(F, n)
     npass = len(sched)
-    log(f"bench: Goldilocks 2^{args.logn} forward NTT, schedule {sched}, steps={args.steps} warmup={args.warmup}")
+    log(f"bench: {F.name} 2^{args.logn} forward NTT, schedule {sched}, steps={args.steps} warmup={args.warmup}")
 
     rng = np.random.default_rng(42)
     host_x = rng.integers(0, F.p, size=n, dtype=np.uint64)
@@ -693,7 +709,7 @@ def main():
     sec_inv = time_device(inv, args.steps, args.warmup, torch)
     log(f"bench: inverse {sec_inv * 1e6:.1f} us/transform, round trip ok={round_trip}")
 
-    alg_bytes = 16 * n
+    alg_bytes = 2 * F.bytes * n
     per_launch = alg_bytes / npass
     achieved = per_launch / (sec / npass) / 1e9
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
@@ -701,14 +717,15 @@ def main():
     traffic = None
     valu_ops = None
     tpath = os.path.join(ROOT, "profiles", "traffic_k_stockham_pass_goldilocks_2p24.json")
-    if args.logn == 24 and os.path.exists(tpath):
+    if args.logn == 24 and F is E.Goldilocks and os.path.exists(tpath):
         with open(tpath) as fh:
             t = json.load(fh)
         if t.get("schedule_log_radix") == sched:
             traffic = t["traffic_bytes_per_launch"]
             valu_ops = sum(x["valu_lane_ops_per_elem"] for x in t["per_pass"])
     result = {
-        "metric": "Goldilocks 2^24 NTT elems/s & %HBM peak; batched-vs-plain speedup, bit-exact",
+        "metric": ("Goldilocks 2^24 NTT elems/s & %HBM peak; batched-vs-plain speedup, bit-exact"
+                   if F is E.Goldilocks else f"{F.name} 2^{args.logn} NTT elems/s & %HBM peak"),
         "value": n / sec,
         "unit": "elems/s",
         "n_gpus": 1,
@@ -720,9 +737,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64 (Goldilocks)",
+        "dtype": "u64 (Goldilocks)" if F.bytes == 8 else f"u32 ({F.name})",
         "data": "synthetic: uniform in [0, p), numpy PCG64 seed 42, 3 rotating HBM-resident buffers",
-        "config": {"workload": f"Goldilocks forward NTT N=2^{args.logn}, natural order in/out",
+        "config": {"workload": f"{F.name} forward NTT N=2^{args.logn}, natural order in/out",
                    "schedule_log_radix": sched, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_stockham_pass (one launch per pass)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -758,7 +775,7 @@ def main():
         torch.cuda.empty_cache()
     if "encoder" in legs:
         result["encoder_256x2p20"] = encoder(E, torch, np, "costmodel" in legs)
-    if not args.no_cpu:
+    if not args.no_cpu and F is E.Goldilocks:  # the CPU restatement timed is the Goldilocks one
         cb, cpu_out = cpu_baseline(host_x, n)
         result["cpu_baseline"] = cb
         if not args.no_extras:
