@@ -263,10 +263,12 @@ omr_status omr_batch_execute_grouped(omr_batch_plan *plan, const void *d_in, voi
 omr_status omr_batch_execute_grouped_graph(omr_batch_plan *plan, const void *d_in, void *d_out,
                                            void *stream);
 /* Largest pass radix (2^5 .. 2^12; OMR_E_SCHEDULE otherwise) the batch
- * planner gives multi-pass vectors of later plans (default 2^9). */
+ * planner gives multi-pass vectors of later plans.  0 (the default) is
+ * automatic: 2^10 for Goldilocks, 2^9 for the 4-byte fields. */
 omr_status omr_batch_set_max_log_radix(int log_radix);
 /* Pass radices the batch planner gives one vector of 2^log_n elements under
- * a radix cap of 2^max_log_radix (0 = the current setting): one pass up to
+ * a radix cap of 2^max_log_radix (0 = the current setting; when that is
+ * automatic, the Goldilocks cap 2^10): one pass up to
  * 2^12, else ceil(log_n / cap) balanced passes, each 2^4 .. 2^12.  Writes up
  * to 9 log radices and the pass count.  Host only (no GPU work). */
 omr_status omr_batch_vector_schedule(uint32_t log_n, int max_log_radix, int *log_radix, int *npass);

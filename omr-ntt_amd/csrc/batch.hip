@@ -1,7 +1,7 @@
 // batch.hip -- batched transforms of vectors of uneven power-of-two sizes.
 //
 // A plan splits every vector into Stockham passes of radix 2^4 .. 2^12 and
-// lists the 4096-element tiles of pass i of all vectors as the tasks of
+// lists the tiles (8192 or 4096 elements) of pass i of all vectors as the tasks of
 // launch i of the persistent kernel k_batch_tiles (batch_kernel.h).  Pass
 // boundaries are kernel boundaries: a pass reads what other workgroups (on
 // other XCDs, with their own L2) wrote in the previous pass, which coarse-
@@ -74,16 +74,16 @@ static hipError_t launch_batch(int f, dim3 grid, hipStream_t s, const BatchTask 
                                const BatchVec *v, const BatchTables *tb, BatchBases b) {
   switch (f) {
     case kBabyBear:
-      hipLaunchKernelGGL(k_batch_tiles<BabyBear>, grid, dim3(kTileThreads), 0, s, t, nt, v, tb, b);
+      hipLaunchKernelGGL(k_batch_tiles<BabyBear>, grid, dim3(batch_tile_threads(f)), 0, s, t, nt, v, tb, b);
       break;
     case kKoalaBear:
-      hipLaunchKernelGGL(k_batch_tiles<KoalaBear>, grid, dim3(kTileThreads), 0, s, t, nt, v, tb, b);
+      hipLaunchKernelGGL(k_batch_tiles<KoalaBear>, grid, dim3(batch_tile_threads(f)), 0, s, t, nt, v, tb, b);
       break;
     case kGoldilocks:
-      hipLaunchKernelGGL(k_batch_tiles<Goldilocks>, grid, dim3(kTileThreads), 0, s, t, nt, v, tb, b);
+      hipLaunchKernelGGL(k_batch_tiles<Goldilocks>, grid, dim3(batch_tile_threads(f)), 0, s, t, nt, v, tb, b);
       break;
     case kTeddyBear:
-      hipLaunchKernelGGL(k_batch_tiles<TeddyBear>, grid, dim3(kTileThreads), 0, s, t, nt, v, tb, b);
+      hipLaunchKernelGGL(k_batch_tiles<TeddyBear>, grid, dim3(batch_tile_threads(f)), 0, s, t, nt, v, tb, b);
       break;
     default:
       return hipErrorInvalidValue;
@@ -91,9 +91,17 @@ static hipError_t launch_batch(int f, dim3 grid, hipStream_t s, const BatchTask 
   return hipGetLastError();
 }
 
-// Largest pass radix the batch planner uses for multi-pass vectors: 2^9
-// keeps >= 8 columns (64 B runs for Goldilocks) per 4096-element tile.
-static int g_batch_max_log_radix = 9;
+// Largest pass radix the batch planner uses for multi-pass vectors; 0 =
+// automatic.  Goldilocks 2^10 (8 columns of its 8192-element tiles, 64-byte
+// runs; vectors up to 2^20 take two passes), BabyBear / KoalaBear 2^9 (16
+// columns, 64-byte runs), TeddyBear 2^9 (8 columns of its 4096-element
+// tiles).  Measured on 4096 vectors of log n ~ U{10..20}: Goldilocks 11.7 ms
+// at 2^9 against 10.5 ms at 2^10, BabyBear 8.4 ms at 2^9 against 8.9 ms at
+// 2^10.
+static int g_batch_max_log_radix = 0;
+static int batch_cap(int field) {
+  return g_batch_max_log_radix ? g_batch_max_log_radix : field == kGoldilocks ? 10 : 9;
+}
 
 // Pass radices of one vector of size 2^logn inside a batch, for a radix cap
 // of 2^cap: one pass up to 2^12, else ceil(logn / cap) balanced passes.
@@ -120,8 +128,8 @@ static bool batch_schedule(uint32_t logn, int cap, std::vector<int> &out) {
 
 extern "C" omr_status omr_batch_set_max_log_radix(int log_radix) try {
   // below 2^5 a balanced split can leave parts of 2^3, which the batch tile
-  // kernel (register radix 16) cannot run
-  if (log_radix < 5 || log_radix > 12) return OMR_E_SCHEDULE;
+  // kernel (register radix 16) cannot run; 0 restores the automatic cap
+  if (log_radix != 0 && (log_radix < 5 || log_radix > 12)) return OMR_E_SCHEDULE;
   g_batch_max_log_radix = log_radix;
   return OMR_OK;
 } catch (const std::bad_alloc &) {
@@ -135,7 +143,7 @@ extern "C" omr_status omr_batch_vector_schedule(uint32_t log_n, int max_log_radi
   if (!log_radix || !npass) return OMR_E_NULL;
   if (log_n < 4 || log_n > 36) return OMR_E_SIZE;
   std::vector<int> sc;
-  if (!batch_schedule(log_n, max_log_radix == 0 ? g_batch_max_log_radix : max_log_radix, sc) || sc.size() > 9)
+  if (!batch_schedule(log_n, max_log_radix == 0 ? batch_cap(kGoldilocks) : max_log_radix, sc) || sc.size() > 9)
     return OMR_E_SCHEDULE;
   *npass = (int)sc.size();
   for (size_t i = 0; i < sc.size(); i++) log_radix[i] = sc[i];
@@ -257,7 +265,7 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
   std::vector<uint64_t> scratch_off(count, 0);
   uint32_t maxpass = 0;
   omr_status st = OMR_OK;
-  const int cap = g_batch_max_log_radix;
+  const int cap = batch_cap(field);
   for (uint32_t i = 0; i < count; i++) {
     const uint32_t logn = items[i].log_n;
     if (!batch_schedule(logn, cap, p->sched[i])) {
@@ -317,9 +325,13 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) cus = prop.multiProcessorCount;
   }
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_tiles<Goldilocks>, kTileThreads, 0) !=
-          hipSuccess || occ < 1)
-    occ = 2;
+  const void *kern = field == kBabyBear    ? (const void *)k_batch_tiles<BabyBear>
+                     : field == kKoalaBear ? (const void *)k_batch_tiles<KoalaBear>
+                     : field == kTeddyBear ? (const void *)k_batch_tiles<TeddyBear>
+                                           : (const void *)k_batch_tiles<Goldilocks>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, batch_tile_threads(field), 0) != hipSuccess ||
+      occ < 1)
+    occ = 1;
 
   for (uint32_t pass = 0; st == OMR_OK && pass < maxpass; pass++) {
     std::vector<BatchTask> tasks;
@@ -342,7 +354,7 @@ extern "C" omr_status omr_batch_plan_create(omr_ctx *ctx, int field, const omr_b
         for (uint32_t q = 0; q <= pass; q++) consumed += sc[q];
         const int logR = sc[pass];
         const int logS = (int)ln - consumed;
-        const int logC = 12 - logR;
+        const int logC = batch_log_tile(field) - logR;
         const int logCols = (int)ln - logR;
         const uint16_t ts = tset_of[std::make_pair(ln, dir)];
         uint8_t flags = 0;

@@ -1,10 +1,11 @@
 // batch_kernel.h -- persistent Stockham tile kernel with a runtime radix,
 // for batches of vectors of uneven power-of-two sizes.
 //
-// Every task is one 4096-element tile of one Stockham pass (the pass
-// semantics of engine.h) of one vector, or of several whole small vectors
-// when a vector has fewer than C columns.  All tasks use 256 threads and a
-// register radix of 16; the pass radix 2^logR (4 <= logR <= 12), the vector
+// Every task is one tile (batch_log_tile: 8192 or 4096 elements) of one
+// Stockham pass (the pass semantics of engine.h) of one vector, or of
+// several whole small vectors when a vector has fewer than C columns.  All
+// tasks use tile / 16 threads and a register radix of 16; the pass radix
+// 2^logR (4 <= logR <= 12), the vector
 // size and the twiddle tables are per-task data, so one kernel serves every
 // size in the batch.  Workgroups walk the task list with a grid stride
 // (tasks are sorted by cost, largest first).
@@ -13,8 +14,14 @@
 
 namespace omr {
 
-constexpr int kTileElems = 4096;
-constexpr int kTileThreads = 256;  // kTileElems / 16
+// log2 of the tile of field id f.  2^13 elements (64 KiB of Goldilocks
+// words): a radix-2^10 pass still has 8 columns, i.e. 64-byte runs, so
+// vectors up to 2^20 take two passes.  TeddyBear keeps 2^12: its kernel
+// (99 VGPRs) runs 4 waves per SIMD with either tile, and in 8-wave
+// workgroups each barrier holds twice the waves (4096 vectors of
+// log n ~ U{10..20}: 10.1 ms with 2^12 tiles, 10.8 ms with 2^13).
+constexpr int batch_log_tile(int field) { return field == kTeddyBear ? 12 : 13; }
+constexpr int batch_tile_threads(int field) { return (1 << batch_log_tile(field)) / 16; }
 
 struct BatchTask {
   uint32_t vec0;   // first vector of the task (index into the pass's vector table)
@@ -74,7 +81,9 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
   const uint32_t logN = d.logN, logR = d.logR, logS = d.logS;
   const uint32_t R = 1u << logR, RE = R >> 4;
   const T *__restrict__ root = pyr + R;  // w_R^m, m < R: one contiguous block
-  const uint32_t logC = 12 - logR;  // C = 4096 / R columns per task
+  constexpr int kLogTile = batch_log_tile(F::ID);
+  constexpr uint32_t kTileElems = 1u << kLogTile, kTileThreads = kTileElems / 16;
+  const uint32_t logC = kLogTile - logR;  // C = kTileElems / R columns per task
   const uint32_t c = tid & ((1u << logC) - 1), t = tid >> logC;
   const uint32_t logCols = logN - logR;  // columns per vector
   const uint32_t logCv = logCols < logC ? logCols : logC;
@@ -108,7 +117,7 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
     // each vector of the task contributes the contiguous block
     // [q0 R, (q0 + Cv) R) of its data, q0 = tile * C
     const uint32_t vblk = 1u << (logCv + logR);
-    for (uint32_t i = tid; i < (uint32_t)kTileElems; i += kTileThreads) {
+    for (uint32_t i = tid; i < kTileElems; i += kTileThreads) {
       const uint32_t uu = i >> (logCv + logR);
       if (uu >= d.nvec) continue;
       const uint32_t ii = i & (vblk - 1);
@@ -202,15 +211,16 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
   }
 }
 
-// 4 waves per SIMD: what the LDS tile allows (4 workgroups of 4 waves per
-// CU).  Goldilocks needs a few dwords of spill for it and still runs 7 %
-// faster than at 3 waves per SIMD without the cap (4096 Goldilocks vectors of log n ~ U{10..20}, measured).
+// 4 waves per SIMD: what the Goldilocks LDS tile allows (2 workgroups of 8
+// waves per CU); Goldilocks fits it in 128 VGPRs without spills.  With
+// 4096-element tiles the cap ran 7 % faster than 3 waves per SIMD without it
+// (4096 Goldilocks vectors of log n ~ U{10..20}).
 template <class F>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, kTileThreads), amdgpu_waves_per_eu(4)))
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, batch_tile_threads(F::ID)), amdgpu_waves_per_eu(4)))
 k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchVec *__restrict__ vecs,
               const BatchTables *__restrict__ tsets, BatchBases bases) {
   using T = typename F::T;
-  __shared__ T lds[lds_padded(kTileElems)];
+  __shared__ T lds[lds_padded(1 << batch_log_tile(F::ID))];
   const uint32_t tid = threadIdx.x;
 
   // the next task's descriptor is loaded while this one runs: its tables,

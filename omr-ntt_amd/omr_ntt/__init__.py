@@ -487,6 +487,8 @@ class BatchPlan:
 
 
 def batch_set_max_log_radix(r):
+    """Radix cap 2^r (5..12) for later batch plans; 0 = automatic (2^10
+    Goldilocks, 2^9 for the 4-byte fields)."""
     _check(lib().omr_batch_set_max_log_radix(r))
 
 

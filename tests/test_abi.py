@@ -166,3 +166,7 @@ def test_batch_vector_schedules_fit_the_tile_kernel():
             E.batch_vector_schedule(16, cap)
     with pytest.raises(E.OmrError):
         E.batch_vector_schedule(3, 9)  # vectors start at 2^4
+    # the automatic cap (0, the default): two passes up to 2^20 for Goldilocks
+    assert E.lib().omr_batch_set_max_log_radix(0) == 0
+    assert E.batch_vector_schedule(20) == [10, 10]
+    assert E.batch_vector_schedule(19, 9) == [7, 6, 6]

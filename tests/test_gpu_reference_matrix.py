@@ -129,7 +129,7 @@ def test_batch_plan_at_radix_cap_5():
             assert np.array_equal(E.from_device(out, F), exp), mode
         plan.close()
     finally:
-        E.batch_set_max_log_radix(9)
+        E.batch_set_max_log_radix(0)  # automatic
 
 
 @pytest.mark.slow
