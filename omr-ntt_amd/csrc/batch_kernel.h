@@ -116,18 +116,32 @@ __device__ __forceinline__ void batch_task(const BatchTask &d, const BatchTables
   } else {
     // each vector of the task contributes the contiguous block
     // [q0 R, (q0 + Cv) R) of its data, q0 = tile * C
-    const uint32_t vblk = 1u << (logCv + logR);
-    for (uint32_t i = tid; i < kTileElems; i += kTileThreads) {
-      const uint32_t uu = i >> (logCv + logR);
-      if (uu >= d.nvec) continue;
-      const uint32_t ii = i & (vblk - 1);
-      const uint32_t jj = ii & (S - 1);
-      const uint32_t k = (ii >> logS) & (R - 1);
-      const uint32_t pp = ii >> (logS + logR);
-      const BatchVec v2 = vecs[d.vec0 + uu];
-      const T *s2 = (const T *)bases.buf[v2.src_buf] + v2.src_off;
-      const uint32_t col = (uu << logCv) + (pp << logS) + jj;
-      lds[lpad((k << logC) + col)] = s2[(d.tile << (logC + logR)) + ii];
+    if (sizeof(T) == 4 && d.nvec == 1) {
+      // one vector (u0 = 0, src is its base): the block is one run of
+      // kTileElems elements, 16 independent loads per thread.  (Goldilocks
+      // stages only passes of stride 2 .. C - 1, rare in batches, and this
+      // form would cost its kernel spills.)
+      const T *blk = src + (d.tile << (logC + logR));
+#pragma unroll
+      for (uint32_t m = 0; m < 16; m++) {
+        const uint32_t i = tid + m * kTileThreads;
+        const uint32_t jj = i & (S - 1), k = (i >> logS) & (R - 1), pp = i >> (logS + logR);
+        lds[lpad((k << logC) + (pp << logS) + jj)] = blk[i];
+      }
+    } else {
+      const uint32_t vblk = 1u << (logCv + logR);
+      for (uint32_t i = tid; i < kTileElems; i += kTileThreads) {
+        const uint32_t uu = i >> (logCv + logR);
+        if (uu >= d.nvec) continue;
+        const uint32_t ii = i & (vblk - 1);
+        const uint32_t jj = ii & (S - 1);
+        const uint32_t k = (ii >> logS) & (R - 1);
+        const uint32_t pp = ii >> (logS + logR);
+        const BatchVec v2 = vecs[d.vec0 + uu];
+        const T *s2 = (const T *)bases.buf[v2.src_buf] + v2.src_off;
+        const uint32_t col = (uu << logCv) + (pp << logS) + jj;
+        lds[lpad((k << logC) + col)] = s2[(d.tile << (logC + logR)) + ii];
+      }
     }
     __syncthreads();
 #pragma unroll

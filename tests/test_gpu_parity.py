@@ -505,13 +505,19 @@ def _batch_items(rng, count, lo, hi, layout_gap=0):
     return items, off
 
 
-@pytest.mark.parametrize("fname", ["Goldilocks", "BabyBear"])
+@pytest.mark.parametrize("fname", ["Goldilocks", "BabyBear", "KoalaBear", "TeddyBear"])
 def test_batched_uneven_sizes_all_modes(fname):
+    """Every execution mode of a batch plan against the oracle, in each field
+    (8192-element batch tiles for Goldilocks / BabyBear / KoalaBear, 4096 for
+    TeddyBear); the 2^19 / 2^20 vectors take the two-pass splits of the
+    default radix cap ([10, 9] / [10, 10] for Goldilocks)."""
     F = field(fname)
     rng = np.random.default_rng(7)
     items, total = _batch_items(rng, 96, 4, 16, layout_gap=3)
     items += [(total, total, 4, 0), (total + 16, total + 16, 13, 1)]
     total += 16 + (1 << 13)
+    items += [(total, total, 19, 0), (total + (1 << 19), total + (1 << 19), 20, 1)]
+    total += (1 << 19) + (1 << 20)
     x = inputs(F, total, 12)
     t_in = E.to_device(x, F)
     plan = E.BatchPlan(F, items)
