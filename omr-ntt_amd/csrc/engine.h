@@ -573,16 +573,44 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
       const T *tlo = (const T *)a.tw_lo, *thi = (const T *)a.tw_hi;
       const uint32_t lomask = (1u << a.lo_bits) - 1;
       const bool one = ((C << LOGR) << logP) <= (1u << a.lo_bits);  // c k' < C N/s
+      auto twN = [&](uint32_t e) -> T {  // w_N^e, e < C N/s <= N
+        return one ? tlo[e] : F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]);
+      };
+      const uint32_t cP = c << logP;
       __syncthreads();  // the last in-tile exchange has been read
+      if (one || sizeof(T) == 8) {  // one or two lookups per element
 #pragma unroll
-      for (int g = 0; g < E / EL; g++) {
+        for (int g = 0; g < E / EL; g++) {
 #pragma unroll
-        for (int kk = 0; kk < EL; kk++) {
-          const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
-          if (krow < nrows) {
-            const uint32_t e = c * (p + (krow << logP));
-            const T w = one ? tlo[e] : F::wmul(tlo[e & lomask], thi[e >> a.lo_bits]);
-            lds[lpad((krow << logC) + c)] = F::mulw(x[g * EL + kk], w);
+          for (int kk = 0; kk < EL; kk++) {
+            const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
+            if (krow < nrows) lds[lpad((krow << logC) + c)] = F::mulw(x[g * EL + kk], twN(c * p + cP * krow));
+          }
+        }
+      } else {
+        // 4-byte fields, two-level table: w_N^(c k') on the thread's rows
+        // krow = kk R/EL + t + g RE is A_g B^kk per row group, A_g =
+        // w_N^(c (p + P (t + g RE))), B = w_N^(c P R/EL): two power chains
+        // (even / odd kk, stepped by B^2), the same two products per element
+        // as the two-level lookup, two lookups per group instead of two per
+        // element.  A/B on one box, N = 2^22 BabyBear, prefixes of 2^16 /
+        // 2^20: 31.8 -> 27.7-28.8 us, 40.4-41.2 -> 37.9 us (2^8, 2^12
+        // unchanged).  Goldilocks, whose products cost 22 VALU slots against
+        // 6, measured mixed (2^16 -5 %, 2^12 and 2^20 +2 %) and keeps the
+        // lookups.
+        const T B = twN(cP * (uint32_t)(R / EL));
+        const T B2 = F::wmul(B, B);
+#pragma unroll
+        for (int g = 0; g < E / EL; g++) {
+          const T A = twN(c * p + cP * (t + (uint32_t)g * RE));
+          T ch[2] = {A, F::wmul(A, B)};
+#pragma unroll
+          for (int kk = 0; kk < EL; kk++) {
+            const uint32_t krow = (uint32_t)kk * (R / EL) + t + g * RE;
+            if (krow < nrows) {  // rows grow with kk: a chain past nrows stops
+              lds[lpad((krow << logC) + c)] = F::mulw(x[g * EL + kk], ch[kk & 1]);
+              if (kk + 2 < EL) ch[kk & 1] = F::wmul(ch[kk & 1], B2);
+            }
           }
         }
       }
