@@ -672,6 +672,26 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
         if (a.flags & kScale) pch[g] = F::wmul(pch[g], (T)a.scale_w);  // an inverse's N^-1, once per element
       }
     }
+    // Decode's output multiplier (post_lo / post_hi: the two-level table of
+    // g^e, e < N) on output oidx = (krow << logNR) + q, 4-byte fields: power
+    // chains per row group, och = g^(((t + g RE) << logNR) + q) stepped by
+    // g^((2 R/EL) << logNR) over even / odd kk, two lookups per group and
+    // the same two products per element as the lookup.
+    T och[E / EL][2], ostep2 = 0;
+    if constexpr (sizeof(T) == 4 && EPI == kEpiGeneral) {
+      if (a.post_lo) {
+        const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
+        const uint32_t lomask = (1u << a.lo_bits) - 1;
+        auto pw = [&](uint32_t e) -> T { return F::wmul(plo[e & lomask], phi[e >> a.lo_bits]); };
+        const T ostep = pw((uint32_t)(R / EL) << logNR);
+        ostep2 = F::wmul(ostep, ostep);
+#pragma unroll
+        for (int g = 0; g < E / EL; g++) {
+          och[g][0] = pw(((t + (uint32_t)g * RE) << logNR) + q);
+          och[g][1] = F::wmul(och[g][0], ostep);
+        }
+      }
+    }
     // WIDE: 64-bit output words (a uniform choice: the loop is compiled twice)
     auto store_all = [&](auto wide_tag) {
       constexpr bool WIDE = decltype(wide_tag)::value;
@@ -688,8 +708,13 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
           if (scale) v = F::mulw(v, sw);
           const uint32_t oidx = (krow << logNR) + q;
           if (EPI == kEpiGeneral && a.post_lo) {
-            const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
-            v = F::mulw(v, F::wmul(plo[oidx & ((1u << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
+            if constexpr (sizeof(T) == 4) {
+              v = F::mulw(v, och[g][kk & 1]);
+              if (kk + 2 < EL) och[g][kk & 1] = F::wmul(och[g][kk & 1], ostep2);
+            } else {
+              const T *plo = (const T *)a.post_lo, *phi = (const T *)a.post_hi;
+              v = F::mulw(v, F::wmul(plo[oidx & ((1u << a.lo_bits) - 1)], phi[oidx >> a.lo_bits]));
+            }
           }
           if (!pruned || (uint64_t)oidx < a.store_limit) {
             if constexpr (WIDE) dstw[oidx] = (uint64_t)v;
