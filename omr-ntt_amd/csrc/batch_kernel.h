@@ -240,11 +240,16 @@ k_batch_tiles(const BatchTask *__restrict__ tasks, uint32_t ntasks, const BatchV
   // the next task's descriptor is loaded while this one runs: its tables,
   // vector entry and data loads then start without a chain of dependent
   // descriptor latencies
-  BatchTask dn = tasks[blockIdx.x < ntasks ? blockIdx.x : 0];
-  for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+  // Each round of G tasks gives XCD x (workgroups b = x mod 8, dealt round
+  // robin) the x-th contiguous eighth, so consecutive tiles of a vector,
+  // whose 64-byte column runs share 128-byte lines, meet in one L2.
+  const uint32_t G = gridDim.x;
+  const uint32_t slot = (G & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+  BatchTask dn = tasks[slot < ntasks ? slot : 0];
+  for (uint32_t ti = slot; ti < ntasks; ti += G) {
     const BatchTask d = dn;
     const BatchTables tb = tsets[d.tset];
-    if (ti + gridDim.x < ntasks) dn = tasks[ti + gridDim.x];
+    if ((uint64_t)ti + G < ntasks) dn = tasks[ti + G];
     if constexpr (F::kPow2Roots) {  // per-task register-DFT constants: forward / inverse roots
       if (tb.unit == 39) batch_task<F, 39>(d, tb, vecs, bases, lds, tid);
       else if (tb.unit == 153) batch_task<F, 153>(d, tb, vecs, bases, lds, tid);
