@@ -440,8 +440,14 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
   load_consts<F, LOGE, U>(wE, rootR, RE, 0);
 
   {
-    // grid = nvec << logTiles tiles; wide tiles (short runs) in XCD order
-    const uint64_t gt = K::WIDE ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    // grid = nvec << logTiles tiles; tiles whose column runs are shorter
+    // than a 128-byte line (wide tiles, small jobs' narrow tiles, Goldilocks
+    // radix >= 2^9) in XCD order, so that the tiles sharing a line meet in one
+    // L2 (A/B on one box: the radix-8 variant at 2^24, [9,9,6] with 64-byte
+    // runs at radix 512, 277.5-279.4 -> 241.3-244.6 us; 128-byte runs keep
+    // the plain order, which measured faster for them)
+    const bool short_runs = K::WIDE || ((uint32_t)sizeof(T) << logC) < 128u;
+    const uint64_t gt = short_runs ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint64_t vec = gt >> logTiles;
     const uint32_t tile = (uint32_t)(gt & ((1ull << logTiles) - 1));
     const uint64_t voff_in = a.vec_off_in ? a.vec_off_in[vec] : vec * a.vec_stride_in;
@@ -593,11 +599,9 @@ __device__ __forceinline__ void stockham_pass_body(const PassArgs &a) {
         // w_N^(c (p + P (t + g RE))), B = w_N^(c P R/EL): two power chains
         // (even / odd kk, stepped by B^2), the same two products per element
         // as the two-level lookup, two lookups per group instead of two per
-        // element.  A/B on one box, N = 2^22 BabyBear, prefixes of 2^16 /
-        // 2^20: 31.8 -> 27.7-28.8 us, 40.4-41.2 -> 37.9 us (2^8, 2^12
-        // unchanged).  Goldilocks, whose products cost 22 VALU slots against
-        // 6, measured mixed (2^16 -5 %, 2^12 and 2^20 +2 %) and keeps the
-        // lookups.
+        // element.  N = 2^22 BabyBear, prefix of 2^16: 31.9-32.2 -> 28.1-28.4
+        // us (2^8, 2^12, 2^20 unchanged).  Goldilocks, whose products cost 22
+        // VALU slots against 6, keeps the lookups.
         const T B = twN(cP * (uint32_t)(R / EL));
         const T B2 = F::wmul(B, B);
 #pragma unroll
