@@ -9,6 +9,12 @@ log2 N = 10 .. min(26, two-adicity), as many vectors as make 512 MB (at least
 two, at most 65535), so that the timed batch streams from HBM rather than the
 256 MB Infinity Cache.
 Partial prefixes: log2 N in {16, 18, 20, 22, 24}, every log2 out_len < log2 N.
+
+    python -m omr_ntt.tune_db --out omr-ntt_amd/tuning/mi355x.tune --partial-only BabyBear,KoalaBear
+
+re-times only the partial-prefix shapes of the named fields, keeps every
+other entry of the table at --out (and of its .json report), and rewrites
+both.
 """
 import argparse
 import json
@@ -19,6 +25,43 @@ import time
 if __package__ in (None, ""):
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 import omr_ntt as E  # noqa: E402
+
+
+PARTIAL_LOG_N = (16, 18, 20, 22, 24)
+
+
+def tune_partials(F, reps, report):
+    for ln in PARTIAL_LOG_N:
+        if ln > F.TWO_ADICITY:
+            continue
+        for lo in range(1, ln):
+            n = 1 << ln
+            c, timed = E.autotune_shape(F, n, 1 << lo, 1, use_model=False, reps=reps)
+            report.append({"field": F.name, "log_n": ln, "out": lo, "class": c, "timed": timed,
+                           "schedule": E.schedule_class(F, n, 1 << lo, c)})
+
+
+def retune_partials(out, names, reps=5):
+    """Re-times the partial-prefix shapes of the named fields on top of the
+    table at `out`; every other entry stays as it is."""
+    fields = [getattr(E, nm) for nm in names]
+    E.context()
+    E.tune_clear()
+    E.tune_load(out)
+    with open(out + ".json") as fh:
+        old = json.load(fh)
+    keep = [r for r in old if not (r["field"] in names and r["out"] < r["log_n"] and "batch" not in r)]
+    report = []
+    t0 = time.time()
+    for F in fields:
+        tune_partials(F, reps, report)
+        print(f"{F.name}: partial shapes re-tuned ({time.time() - t0:.0f} s)", flush=True)
+    before = {(r["field"], r["log_n"], r["out"]): r["class"] for r in old if r not in keep}
+    changed = sum(1 for r in report if before.get((r["field"], r["log_n"], r["out"])) != r["class"])
+    E.tune_save(out)
+    with open(out + ".json", "w") as fh:
+        json.dump(keep + report, fh, indent=0)
+    print(f"wrote {out}: {len(report)} partial shapes re-timed, {changed} changed class")
 
 
 def build(out, max_log=26, reps=5, partial=True):
@@ -42,14 +85,7 @@ def build(out, max_log=26, reps=5, partial=True):
                            "schedule": E.schedule_class(F, n, n, c)})
             print(f"{F.name} 2^{ln} x {batch}: class {c} {report[-1]['schedule']}", flush=True)
         if partial:
-            for ln in (16, 18, 20, 22, 24):
-                if ln > F.TWO_ADICITY:
-                    continue
-                for lo in range(1, ln):
-                    n = 1 << ln
-                    c, timed = E.autotune_shape(F, n, 1 << lo, 1, use_model=False, reps=reps)
-                    report.append({"field": F.name, "log_n": ln, "out": lo, "class": c, "timed": timed,
-                                   "schedule": E.schedule_class(F, n, 1 << lo, c)})
+            tune_partials(F, reps, report)
             print(f"{F.name}: partial shapes tuned ({time.time() - t0:.0f} s)", flush=True)
     E.tune_save(out)
     with open(out + ".json", "w") as fh:
@@ -63,5 +99,10 @@ if __name__ == "__main__":
     ap.add_argument("--max-log", type=int, default=26)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--no-partial", action="store_true")
+    ap.add_argument("--partial-only", default=None, metavar="FIELDS",
+                    help="comma-separated field names: re-time only their partial-prefix shapes")
     a = ap.parse_args()
-    build(a.out, a.max_log, a.reps, not a.no_partial)
+    if a.partial_only:
+        retune_partials(a.out, a.partial_only.split(","), a.reps)
+    else:
+        build(a.out, a.max_log, a.reps, not a.no_partial)
